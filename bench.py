@@ -1,0 +1,173 @@
+#!/usr/bin/env python
+"""Traced rays/s of one 1280x720 frame at LOD 4 on one MI355X.
+
+A step is one frame: box entry, sphere march and finite-difference normals
+for all 921 600 rays (SphereTracer semantics), run by the persistent HIP
+kernel.  Rays are resident in HBM before the timed region starts.  Prints
+one JSON line (see the keys below); --gpus above 1 is refused, this path is
+measured on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, datasheet
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def bytes_per_query(lod, fdim=32, elem=4):
+    """Algorithmic bytes of one SDF query at `lod`: 8 corners x C channels per
+    level for levels 0..lod, plus 12 B of position in and 4 B of distance out."""
+    return (lod + 1) * 8 * fdim * elem + 16
+
+
+def load_traffic(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def cpu_baseline(net, o, d, lod, num_steps, sample_rays, threads):
+    """The reference's PyTorch path restated (oracle/torch_ref.py) on a strided
+    sample of the frame's rays, on the host cores."""
+    import torch
+
+    from oracle.torch_ref import TorchOctreeSDF, sphere_trace
+
+    torch.set_num_threads(threads)
+    ref = TorchOctreeSDF.from_module(net)
+    ref.lod = lod
+    n = o.shape[0]
+    stride = max(1, n // sample_rays)
+    oc = o[::stride].cpu().contiguous()
+    dc = d[::stride].cpu().contiguous()
+    t0 = time.perf_counter()
+    sphere_trace(ref, oc, dc, num_steps=num_steps)
+    dt = time.perf_counter() - t0
+    return {"value": oc.shape[0] / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{oc.shape[0]} rays (every {stride}th ray of the same frame), torch {torch.__version__}, "
+                      f"{threads} threads, {cpu_model()}, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--form", choices=["fp32", "mixed"], default="fp32")
+    ap.add_argument("--width", type=int, default=1280)
+    ap.add_argument("--height", type=int, default=720)
+    ap.add_argument("--lod", type=int, default=4)
+    ap.add_argument("--num-steps", type=int, default=128)
+    ap.add_argument("--cpu-sample", type=int, default=65536, help="rays in the CPU baseline sample (0 = skip)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures this path on one GPU; --gpus above 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from nglod_amd import SphereTracer, _lib, workloads
+
+    dev = torch.device("cuda:0")
+    net = workloads.bench_model(device=dev)
+    net.lod = args.lod
+    o, d = workloads.camera_rays(args.width, args.height, eye=(0.0, 0.0, 3.0), device=dev)
+    n = o.shape[0]
+    tracer = SphereTracer(num_steps=args.num_steps, min_dis=3e-4, camera_clamp=[-5, 10], step_size=1.0,
+                          form=args.form)
+
+    for _ in range(args.warmup):
+        tracer(net, o, d)
+    torch.cuda.synchronize()
+
+    # Timed region: K frames back to back.
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    evals = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = []
+    for k in range(args.steps):
+        starts[k].record()
+        rb = tracer(net, o, d)
+        ends[k].record()
+        outs.append(tracer.last_evals)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+    evals = int(outs[-1].item())
+    hits = int(rb.hit.sum().item())
+
+    ms_per_step = wall * 1e3 / args.steps
+    rays_per_s = n * args.steps / wall
+    elem = 2 if args.form == "mixed" else 4
+    bq = bytes_per_query(args.lod, elem=elem)
+    alg_bytes = evals * bq + n * (24 + 29)
+    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    kname = "trace_dense_fp32_kernel" if args.form == "fp32" else "trace_dense_mixed_kernel"
+    tr = load_traffic(f"{kname}_{args.width}x{args.height}_lod{args.lod}.traffic.json")
+
+    line = {
+        "metric": "traced rays/s @1280x720 LOD4",
+        "value": rays_per_s,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.form == "fp32" else "f16-grid/bf16-mfma",
+        "data": "synthetic",
+        "config": {
+            "workload": f"sphere trace {args.width}x{args.height} pinhole frame, LOD {args.lod}, "
+                        f"max {args.num_steps} steps, min_dis 3e-4, FD normals; {workloads.BENCH_MODEL_NAME}",
+            "rays": n,
+            "lod": args.lod,
+            "form": args.form,
+            "sdf_evals_per_frame": evals,
+            "hits": hits,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": (tr or {}).get("bytes_per_launch"),
+            "kernel": kname,
+            "kernel_ms": kernel_ms,
+            "bytes_per_query": bq,
+            "algorithmic_bytes": alg_bytes,
+        },
+    }
+    if args.cpu_sample > 0:
+        threads = min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(net, o, d, args.lod, args.num_steps, args.cpu_sample, threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,141 @@
+/*
+ * libnglod_hip.so — C ABI of the MI355X (gfx950) NGLOD neural-SDF query and
+ * sphere-tracing path.
+ *
+ * Every entry point takes device pointers the caller allocated, element
+ * counts, and a HIP stream handle (hipStream_t passed as void*; NULL = the
+ * null stream).  Calls are asynchronous on that stream unless stated.  The
+ * library never frees caller memory.  Return value: NGLOD_OK (0) or a
+ * negative status; no C++ exception crosses this boundary.
+ *
+ * Each declaration names the reference interface it replaces (file:line,
+ * paths relative to the nv-tlabs/nglod repository root).
+ */
+#ifndef NGLOD_HIP_H
+#define NGLOD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NGLOD_ABI_VERSION 1
+#define NGLOD_MAX_LODS 8
+
+/* status codes */
+#define NGLOD_OK 0
+#define NGLOD_ERR_ARG (-1)         /* null pointer, negative size, bad lod */
+#define NGLOD_ERR_UNSUPPORTED (-2) /* shape the kernels are not built for */
+#define NGLOD_ERR_CAPACITY (-4)    /* output buffer too small; see *needed */
+#define NGLOD_ERR_HIP (-5)         /* HIP runtime error (launch, device) */
+#define NGLOD_ERR_WORKSPACE (-6)   /* workspace smaller than *_workspace_size */
+
+/* arithmetic forms */
+#define NGLOD_FP32 0  /* FP32 grids, FP32 decoder */
+#define NGLOD_MIXED 1 /* FP16 grids, BF16 MFMA decoder with FP32 accumulation */
+
+/* Dense multi-LOD model (OctreeSDF, sdf-net/lib/models/OctreeSDF.py:59-88).
+ * Device pointers.  grid[i] is corner-major: element
+ * ((z*(r+1) + y)*(r+1) + x)*feature_dim + c  is  fm[0, c, z, y, x] of the
+ * reference FeatureVolume (OctreeSDF.py:43) — i.e. torch channels_last_3d.
+ * Decoder i: w0[i] [hidden][din], b0[i] [hidden], w1[i] [hidden], b1[i] [1],
+ * din = feature_dim + (pos_invariant ? 0 : 3), xyz first (OctreeSDF.py:114-115).
+ * grid_h[i] / w0_bf16[i] are only read by NGLOD_MIXED (see nglod_pack_mixed). */
+typedef struct nglod_model {
+    int32_t num_lods;      /* feature volumes in use, 1..NGLOD_MAX_LODS */
+    int32_t feature_dim;   /* C; kernels are built for 32 */
+    int32_t hidden_dim;    /* H; kernels are built for 128 */
+    int32_t pos_invariant; /* 1: decoder input is the feature only */
+    int32_t num_decoders;  /* 1 (joint_decoder) or num_lods */
+    int32_t res[NGLOD_MAX_LODS];
+    const float* grid[NGLOD_MAX_LODS];
+    const float* w0[NGLOD_MAX_LODS];
+    const float* b0[NGLOD_MAX_LODS];
+    const float* w1[NGLOD_MAX_LODS];
+    const float* b1[NGLOD_MAX_LODS];
+    const uint16_t* grid_h[NGLOD_MAX_LODS];  /* FP16 copy of grid[i] */
+    const uint16_t* w0_bf16[NGLOD_MAX_LODS]; /* BF16 W0 in MFMA fragment order */
+} nglod_model_t;
+
+/* Sphere-tracer parameters (BaseTracer.__init__, sdf-net/lib/tracer/BaseTracer.py:29-45;
+ * defaults sdf-net/lib/options.py:209-221). */
+typedef struct nglod_trace_params {
+    int32_t num_steps;  /* >= 1 */
+    double min_dis;     /* hit threshold; oscillation threshold is 3*min_dis */
+    double step_size;   /* scales every in-loop SDF step */
+    double far_clip;    /* camera_clamp[1] */
+    int32_t normals;    /* 1: central-difference normals at hit points */
+    int32_t refine_steps; /* NGLOD_MIXED only: FP32 steps re-evaluated at the end */
+} nglod_trace_params_t;
+
+int nglod_abi_version(void);
+
+/* Last HIP error string recorded by this thread (static storage). */
+const char* nglod_last_error(void);
+
+/* Ray vs [-1,1]^3 box entry.  Replaces sol_nglod.aabb
+ * (sdf-net/lib/extensions/sol_nglod/sol_nglod_kernel.cu:161-188, bound at :190-192,
+ * called from sdf-net/lib/tracer/SphereTracer.py:53).
+ * ray_o, ray_d: [n][3] fp32.  Writes x [n][3] (entry point, or ray_o when no
+ * entry), t [n] (0 when no entry), hit [n] (0/1).  Bit-exact with the reference
+ * kernel's arithmetic (double reciprocal, explicit fmaf). */
+int nglod_aabb(const float* ray_o, const float* ray_d, int64_t n,
+               float* x, float* t, uint8_t* hit, void* stream);
+
+/* One feature volume sampled trilinearly: FeatureVolume.forward
+ * (sdf-net/lib/models/OctreeSDF.py:46-57; F.grid_sample with
+ * align_corners=True, padding_mode='border').  grid: corner-major
+ * [(res+1)^3][feature_dim] fp32; x [n][3]; out [n][feature_dim]. */
+int nglod_grid_sample(const float* grid, int32_t res, int32_t feature_dim, const float* x, int64_t n,
+                      float* out, void* stream);
+
+/* Fused multi-LOD gather + decoder: out[i] = OctreeSDF.sdf(x[i], lod)
+ * (sdf-net/lib/models/OctreeSDF.py:94-155; net(x) via BaseLOD.forward,
+ * sdf-net/lib/models/BaseLOD.py:37-41).  Features never leave registers.
+ * x [n][3] fp32, out [n] fp32.  form = NGLOD_FP32 or NGLOD_MIXED. */
+int nglod_sdf_fwd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
+                  int32_t form, float* out, void* stream);
+
+/* Every level's prediction, OctreeSDF.sdf(x, return_lst=True)[0..lod]
+ * (OctreeSDF.py:148-153).  out is [lod+1][n] fp32.  FP32 form only. */
+int nglod_sdf_fwd_all(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
+                      float* out, void* stream);
+
+/* Workspace bytes nglod_trace_dense needs (ray queue counters + statistics). */
+size_t nglod_trace_dense_workspace_size(void);
+
+/* Persistent sphere tracer with SphereTracer semantics
+ * (sdf-net/lib/tracer/SphereTracer.py:41-132 incl. box entry :53, FD normals
+ * :128-130 via sdf-net/lib/diffutils.py:61-70).  One kernel: box entry, march,
+ * retirement and refill of finished rays, normals.  Outputs x [n][3] (the
+ * position of the last evaluation, which lags t by one step as in the
+ * reference), t [n], hit [n], normal [n][3] (zeros for misses).  Per-ray
+ * arithmetic does not depend on which other rays are in the launch.
+ * evals_out (device, may be NULL): total SDF evaluations, uint64.
+ * ray_steps (device, may be NULL): per ray, the in-loop SDF evaluations
+ * (the reference's loop trips in which the ray was active), int32 [n]. */
+int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const float* ray_d,
+                      int64_t n, int32_t lod, const nglod_trace_params_t* params, int32_t form,
+                      float* x, float* t, uint8_t* hit, float* normal,
+                      void* workspace, size_t workspace_bytes, uint64_t* evals_out,
+                      int32_t* ray_steps, void* stream);
+
+/* Central-difference normals h = 1/192 then normalize(eps=1e-5)
+ * (sdf-net/lib/diffutils.py:61-70, SphereTracer.py:128-130) at points x [n][3]. */
+int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
+                     int32_t form, float* normal, void* stream);
+
+/* Converts an FP32 model's grids to FP16 and W0 to the BF16 MFMA fragment
+ * layout used by NGLOD_MIXED.  grid_h[i] must hold (res[i]+1)^3*feature_dim
+ * uint16, w0_bf16[i] hidden*32 uint16 (pos_invariant) or hidden*64 (xyz
+ * input).  Replaces the half-precision export of SOL_NGLOD
+ * (sdf-net/lib/models/SOL_NGLOD.py:33-46), kept on device. */
+int nglod_pack_mixed(const nglod_model_t* model, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NGLOD_HIP_H */

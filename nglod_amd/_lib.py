@@ -1,0 +1,136 @@
+"""ctypes binding of libnglod_hip.so (the C ABI declared in include/nglod_hip.h).
+
+The library is the only compute path of this package: when it is missing the
+import of any op raises, there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnglod_hip.so")
+
+MAX_LODS = 8
+
+OK = 0
+ERR_ARG = -1
+ERR_UNSUPPORTED = -2
+ERR_CAPACITY = -4
+ERR_HIP = -5
+ERR_WORKSPACE = -6
+
+FP32 = 0
+MIXED = 1
+
+_STATUS = {
+    ERR_ARG: "bad argument",
+    ERR_UNSUPPORTED: "unsupported shape",
+    ERR_CAPACITY: "capacity exceeded",
+    ERR_HIP: "HIP runtime error",
+    ERR_WORKSPACE: "workspace too small",
+}
+
+_p = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+
+
+class NglodModel(ctypes.Structure):
+    """Mirror of nglod_model_t."""
+
+    _fields_ = [
+        ("num_lods", _i32),
+        ("feature_dim", _i32),
+        ("hidden_dim", _i32),
+        ("pos_invariant", _i32),
+        ("num_decoders", _i32),
+        ("res", _i32 * MAX_LODS),
+        ("grid", _p * MAX_LODS),
+        ("w0", _p * MAX_LODS),
+        ("b0", _p * MAX_LODS),
+        ("w1", _p * MAX_LODS),
+        ("b1", _p * MAX_LODS),
+        ("grid_h", _p * MAX_LODS),
+        ("w0_bf16", _p * MAX_LODS),
+    ]
+
+
+class NglodTraceParams(ctypes.Structure):
+    """Mirror of nglod_trace_params_t."""
+
+    _fields_ = [
+        ("num_steps", _i32),
+        ("min_dis", ctypes.c_double),
+        ("step_size", ctypes.c_double),
+        ("far_clip", ctypes.c_double),
+        ("normals", _i32),
+        ("refine_steps", _i32),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/nglod_hip.h declares.
+SIGNATURES = {
+    "nglod_abi_version": (ctypes.c_int, []),
+    "nglod_last_error": (ctypes.c_char_p, []),
+    "nglod_aabb": (ctypes.c_int, [_p, _p, _i64, _p, _p, _p, _p]),
+    "nglod_grid_sample": (ctypes.c_int, [_p, _i32, _i32, _p, _i64, _p, _p]),
+    "nglod_sdf_fwd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _p, _p]),
+    "nglod_sdf_fwd_all": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _p, _p]),
+    "nglod_trace_dense_workspace_size": (_sz, []),
+    "nglod_trace_dense": (
+        ctypes.c_int,
+        [ctypes.POINTER(NglodModel), _p, _p, _i64, _i32, ctypes.POINTER(NglodTraceParams), _i32,
+         _p, _p, _p, _p, _p, _sz, _p, _p, _p],
+    ),
+    "nglod_normals_fd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _p, _p]),
+    "nglod_pack_mixed": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads libnglod_hip.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m nglod_amd.build` "
+            "(the HIP library is the only compute path)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class NglodError(RuntimeError):
+    pass
+
+
+def check(rc, what):
+    if rc != OK:
+        msg = load().nglod_last_error().decode(errors="replace")
+        raise NglodError(f"{what} failed ({rc}: {_STATUS.get(rc, 'error')}): {msg}")
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def stream_of(t):
+    """Raw hipStream_t of torch's current stream on t's device."""
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def require_gpu(*tensors):
+    for t in tensors:
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise ValueError("nglod_amd ops run on the GPU: pass tensors on a cuda (HIP) device")

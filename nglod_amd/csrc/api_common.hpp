@@ -1,0 +1,86 @@
+// Host-side helpers shared by the C-ABI translation units of libnglod_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "../../include/nglod_hip.h"
+#include "device_common.hpp"
+
+namespace nglod {
+
+void set_last_error(const char* msg);
+
+// Returns NGLOD_ERR_HIP and records the message if the last launch failed.
+inline int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        char buf[256];
+        snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
+        set_last_error(buf);
+        return NGLOD_ERR_HIP;
+    }
+    return NGLOD_OK;
+}
+
+inline int fail(int code, const char* msg) {
+    set_last_error(msg);
+    return code;
+}
+
+inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// Validates the model for `lod` / `form` and fills the kernel argument copy.
+inline int make_model_args(const nglod_model_t* m, int lod, int form, ModelArgs* a) {
+    if (!m) return fail(NGLOD_ERR_ARG, "model is NULL");
+    if (m->num_lods < 1 || m->num_lods > NGLOD_MAX_LODS) return fail(NGLOD_ERR_ARG, "num_lods out of range");
+    if (lod < 0 || lod >= m->num_lods) return fail(NGLOD_ERR_ARG, "lod out of range");
+    if (m->feature_dim != kC || m->hidden_dim != kH)
+        return fail(NGLOD_ERR_UNSUPPORTED, "kernels are built for feature_dim 32, hidden_dim 128");
+    if (m->num_decoders != 1 && m->num_decoders != m->num_lods)
+        return fail(NGLOD_ERR_ARG, "num_decoders must be 1 or num_lods");
+    memset(a, 0, sizeof(*a));
+    a->num_lods = m->num_lods;
+    a->pos_inv = m->pos_invariant ? 1 : 0;
+    a->num_dec = m->num_decoders;
+    for (int i = 0; i < m->num_lods; ++i) {
+        if (m->res[i] < 1 || m->res[i] > 1024) return fail(NGLOD_ERR_ARG, "grid resolution out of range");
+        a->res[i] = m->res[i];
+        a->grid[i] = m->grid[i];
+        a->grid_h[i] = m->grid_h[i];
+        if (i <= lod) {
+            if (form == NGLOD_FP32 && (!m->grid[i] || !aligned16(m->grid[i])))
+                return fail(NGLOD_ERR_ARG, "grid pointer NULL or not 16-byte aligned");
+            if (form == NGLOD_MIXED && (!m->grid_h[i] || !aligned16(m->grid_h[i])))
+                return fail(NGLOD_ERR_ARG, "fp16 grid pointer NULL or not 16-byte aligned");
+        }
+    }
+    for (int i = 0; i < m->num_decoders; ++i) {
+        a->w0[i] = m->w0[i];
+        a->b0[i] = m->b0[i];
+        a->w1[i] = m->w1[i];
+        a->b1[i] = m->b1[i];
+        a->w0_bf16[i] = m->w0_bf16[i];
+        if (!m->w0[i] || !m->b0[i] || !m->w1[i] || !m->b1[i]) return fail(NGLOD_ERR_ARG, "decoder pointer NULL");
+    }
+    if (form == NGLOD_MIXED) {
+        const int dec = m->num_decoders == 1 ? 0 : lod;
+        if (!m->w0_bf16[dec] || !aligned16(m->w0_bf16[dec])) return fail(NGLOD_ERR_ARG, "bf16 W0 not packed");
+    } else if (form != NGLOD_FP32) {
+        return fail(NGLOD_ERR_ARG, "unknown form");
+    }
+    return NGLOD_OK;
+}
+
+inline int decoder_of(const nglod_model_t* m, int lod) { return m->num_decoders == 1 ? 0 : lod; }
+
+int device_cu_count();
+
+struct TraceArgs;
+// Mixed-precision launchers (mixed.hip).
+int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out, hipStream_t s);
+int launch_normals_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* nrm, hipStream_t s);
+int launch_trace_mixed(const TraceArgs& a, int refine_steps, hipStream_t s);
+
+}  // namespace nglod

@@ -1,0 +1,336 @@
+// gfx950 kernels and C ABI for the dense NGLOD query path:
+//   nglod_aabb          box entry                      (sol_nglod_kernel.cu:77-188)
+//   nglod_grid_sample   one FeatureVolume          (OctreeSDF.py:46-57)
+//   nglod_sdf_fwd       fused multi-LOD gather+decoder (OctreeSDF.py:94-155)
+//   nglod_sdf_fwd_all   every level's prediction       (OctreeSDF.py:148-153)
+//   nglod_normals_fd    central-difference normals     (diffutils.py:61-70)
+//   nglod_trace_dense   persistent sphere tracer       (SphereTracer.py:41-132)
+#include <stdio.h>
+
+#include "api_common.hpp"
+#include "trace_common.hpp"
+
+namespace nglod {
+
+static thread_local char g_last_error[512] = "";
+void set_last_error(const char* msg) {
+    strncpy(g_last_error, msg, sizeof(g_last_error) - 1);
+    g_last_error[sizeof(g_last_error) - 1] = 0;
+}
+
+int device_cu_count() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cached[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cached[dev] = n;
+    }
+    return cached[dev];
+}
+
+// ---------------------------------------------------------------------------
+__global__ void fill_u32_kernel(uint32_t* p, int n, uint32_t v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void aabb_kernel(const float* __restrict__ o, const float* __restrict__ d,
+                                                   int64_t n, float* __restrict__ x, float* __restrict__ t,
+                                                   uint8_t* __restrict__ hit) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const BoxEntry e = box_entry(o[3 * i], o[3 * i + 1], o[3 * i + 2], d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+    x[3 * i] = e.x;
+    x[3 * i + 1] = e.y;
+    x[3 * i + 2] = e.z;
+    t[i] = e.t;
+    hit[i] = e.hit ? 1 : 0;
+}
+
+// One feature volume: out[i][c] = trilinear sample (FeatureVolume.forward,
+// sdf-net/lib/models/OctreeSDF.py:46-51).
+__global__ __launch_bounds__(256) void grid_sample_kernel(const float* __restrict__ grid, int r,
+                                                          const float* __restrict__ x, int64_t n,
+                                                          float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int S = r + 1;
+    const AxisW ax = axis_weights(x[3 * i], r);
+    const AxisW ay = axis_weights(x[3 * i + 1], r);
+    const AxisW az = axis_weights(x[3 * i + 2], r);
+    const float* base = grid + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
+    const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
+    const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
+    const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
+    float f[kC];
+#pragma unroll
+    for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+    corner_fma(f, base, wxy00 * az.w0);
+    corner_fma(f, base + sx, wxy10 * az.w0);
+    corner_fma(f, base + sy, wxy01 * az.w0);
+    corner_fma(f, base + sy + sx, wxy11 * az.w0);
+    corner_fma(f, base + sz, wxy00 * az.w1);
+    corner_fma(f, base + sz + sx, wxy10 * az.w1);
+    corner_fma(f, base + sz + sy, wxy01 * az.w1);
+    corner_fma(f, base + sz + sy + sx, wxy11 * az.w1);
+    float4* o4 = reinterpret_cast<float4*>(out + i * kC);
+#pragma unroll
+    for (int v = 0; v < kC / 4; ++v) o4[v] = make_float4(f[4 * v], f[4 * v + 1], f[4 * v + 2], f[4 * v + 3]);
+}
+
+template <bool POSINV>
+__global__ __launch_bounds__(256) void sdf_fp32_kernel(ModelArgs m, int lod, int dec, const float* __restrict__ x,
+                                                       int64_t n, float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = sdf_fp32<POSINV>(m, lod, dec, x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+}
+
+template <bool POSINV>
+__global__ __launch_bounds__(256) void sdf_all_fp32_kernel(ModelArgs m, int lod, const float* __restrict__ x,
+                                                           int64_t n, float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float px = x[3 * i], py = x[3 * i + 1], pz = x[3 * i + 2];
+    float f[kC];
+#pragma unroll
+    for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+    for (int l = 0; l <= lod; ++l) {
+        const int r = m.res[l];
+        const int S = r + 1;
+        const AxisW ax = axis_weights(px, r);
+        const AxisW ay = axis_weights(py, r);
+        const AxisW az = axis_weights(pz, r);
+        const float* base = m.grid[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
+        const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
+        const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
+        const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
+        corner_fma(f, base, wxy00 * az.w0);
+        corner_fma(f, base + sx, wxy10 * az.w0);
+        corner_fma(f, base + sy, wxy01 * az.w0);
+        corner_fma(f, base + sy + sx, wxy11 * az.w0);
+        corner_fma(f, base + sz, wxy00 * az.w1);
+        corner_fma(f, base + sz + sx, wxy10 * az.w1);
+        corner_fma(f, base + sz + sy, wxy01 * az.w1);
+        corner_fma(f, base + sz + sy + sx, wxy11 * az.w1);
+        const int dec = m.num_dec == 1 ? 0 : l;
+        out[(int64_t)l * n + i] = decode_fp32<POSINV>(m, dec, px, py, pz, f);
+    }
+}
+
+template <bool POSINV>
+__global__ __launch_bounds__(256) void normals_fp32_kernel(ModelArgs m, int lod, int dec, const float* __restrict__ x,
+                                                           int64_t n, float* __restrict__ nrm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float p[3] = {x[3 * i], x[3 * i + 1], x[3 * i + 2]};
+    float g[3];
+#pragma unroll 1
+    for (int a = 0; a < 3; ++a) {
+        float pp[3] = {p[0], p[1], p[2]}, pm[3] = {p[0], p[1], p[2]};
+        pp[a] = p[a] + kFdH;
+        pm[a] = p[a] - kFdH;
+        const float fp = sdf_fp32<POSINV>(m, lod, dec, pp[0], pp[1], pp[2]);
+        const float fm = sdf_fp32<POSINV>(m, lod, dec, pm[0], pm[1], pm[2]);
+        g[a] = (fp - fm) / kFdTwoH;
+    }
+    normalize_grad(g[0], g[1], g[2], nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+}
+
+template <bool POSINV>
+__global__ __launch_bounds__(256) void trace_dense_fp32_kernel(TraceArgs a) {
+    const uint32_t lane = lane_id();
+    int q = blockIdx.x % kQueues;
+    int tried = 0;
+    bool drained = false;
+    RayState s;
+    s.ray = -1;
+    s.phase = PH_IDLE;
+    uint32_t nevals = 0;
+    for (;;) {
+        // Refill idle lanes until every lane is busy or all queues are drained.
+        for (;;) {
+            const bool need = (s.phase == PH_IDLE) && !drained;
+            const uint64_t mask = __ballot(need);
+            if (mask == 0) break;
+            const int cnt = __popcll(mask);
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&a.queue[q], (uint32_t)cnt);
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+            const int64_t qsize = a.qend[q] - a.qbeg[q];
+            if (need) {
+                const int64_t idx = (int64_t)base + (int64_t)mask_rank(mask);
+                if (idx < qsize) s.start(a, a.qbeg[q] + idx);
+            }
+            if ((int64_t)base + cnt >= qsize) {
+                q = (q + 1) % kQueues;
+                if (++tried == kQueues) drained = true;
+            }
+        }
+        if (__ballot(s.phase != PH_IDLE) == 0) break;
+        if (s.phase != PH_IDLE) {
+            const float d = sdf_fp32<POSINV>(a.m, a.lod, a.dec, s.qx, s.qy, s.qz);
+            ++nevals;
+            s.consume(a, d);
+        }
+    }
+    if (a.evals) {
+        unsigned long long tot = nevals;
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+        if (lane == 0) atomicAdd(a.evals, tot);
+    }
+}
+
+template <typename K>
+static int persistent_grid(K kernel, int64_t n, int block) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu <= 0) per_cu = 4;
+    const int64_t want = (n + block - 1) / block;
+    const int64_t cap = (int64_t)device_cu_count() * per_cu;
+    return (int)(want < cap ? (want < 1 ? 1 : want) : cap);
+}
+
+void split_queues(int64_t n, int64_t* beg, int64_t* end) {
+    for (int i = 0; i < kQueues; ++i) {
+        beg[i] = n * i / kQueues;
+        end[i] = n * (i + 1) / kQueues;
+    }
+}
+
+}  // namespace nglod
+
+using namespace nglod;
+
+extern "C" {
+
+int nglod_abi_version(void) { return NGLOD_ABI_VERSION; }
+
+const char* nglod_last_error(void) { return g_last_error; }
+
+int nglod_aabb(const float* ray_o, const float* ray_d, int64_t n, float* x, float* t, uint8_t* hit, void* stream) {
+    if (n < 0) return fail(NGLOD_ERR_ARG, "n < 0");
+    if (n == 0) return NGLOD_OK;
+    if (!ray_o || !ray_d || !x || !t || !hit) return fail(NGLOD_ERR_ARG, "NULL pointer");
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(aabb_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, ray_o, ray_d, n, x, t, hit);
+    return check_launch("aabb_kernel");
+}
+
+int nglod_grid_sample(const float* grid, int32_t res, int32_t feature_dim, const float* x, int64_t n, float* out,
+                      void* stream) {
+    if (feature_dim != kC) return fail(NGLOD_ERR_UNSUPPORTED, "kernels are built for feature_dim 32");
+    if (res < 1 || res > 1024) return fail(NGLOD_ERR_ARG, "res out of range");
+    if (n < 0) return fail(NGLOD_ERR_ARG, "n < 0");
+    if (n == 0) return NGLOD_OK;
+    if (!grid || !x || !out || !aligned16(grid) || !aligned16(out)) return fail(NGLOD_ERR_ARG, "NULL or misaligned pointer");
+    const int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(grid_sample_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, grid, res, x, n, out);
+    return check_launch("grid_sample_kernel");
+}
+
+int nglod_sdf_fwd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod, int32_t form, float* out,
+                  void* stream) {
+    ModelArgs m;
+    int rc = make_model_args(model, lod, form, &m);
+    if (rc) return rc;
+    if (n < 0) return fail(NGLOD_ERR_ARG, "n < 0");
+    if (n == 0) return NGLOD_OK;
+    if (!x || !out) return fail(NGLOD_ERR_ARG, "NULL pointer");
+    const int dec = decoder_of(model, lod);
+    if (form == NGLOD_MIXED) return launch_sdf_mixed(m, lod, dec, x, n, out, (hipStream_t)stream);
+    const int64_t blocks = (n + 255) / 256;
+    if (m.pos_inv)
+        hipLaunchKernelGGL(sdf_fp32_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, dec, x, n, out);
+    else
+        hipLaunchKernelGGL(sdf_fp32_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, dec, x, n, out);
+    return check_launch("sdf_fp32_kernel");
+}
+
+int nglod_sdf_fwd_all(const nglod_model_t* model, const float* x, int64_t n, int32_t lod, float* out, void* stream) {
+    ModelArgs m;
+    int rc = make_model_args(model, lod, NGLOD_FP32, &m);
+    if (rc) return rc;
+    if (n < 0) return fail(NGLOD_ERR_ARG, "n < 0");
+    if (n == 0) return NGLOD_OK;
+    if (!x || !out) return fail(NGLOD_ERR_ARG, "NULL pointer");
+    const int64_t blocks = (n + 255) / 256;
+    if (m.pos_inv)
+        hipLaunchKernelGGL(sdf_all_fp32_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, x, n, out);
+    else
+        hipLaunchKernelGGL(sdf_all_fp32_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, x, n, out);
+    return check_launch("sdf_all_fp32_kernel");
+}
+
+int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod, int32_t form, float* normal,
+                     void* stream) {
+    ModelArgs m;
+    int rc = make_model_args(model, lod, form, &m);
+    if (rc) return rc;
+    if (n < 0) return fail(NGLOD_ERR_ARG, "n < 0");
+    if (n == 0) return NGLOD_OK;
+    if (!x || !normal) return fail(NGLOD_ERR_ARG, "NULL pointer");
+    const int dec = decoder_of(model, lod);
+    if (form == NGLOD_MIXED) return launch_normals_mixed(m, lod, dec, x, n, normal, (hipStream_t)stream);
+    const int64_t blocks = (n + 255) / 256;
+    if (m.pos_inv)
+        hipLaunchKernelGGL(normals_fp32_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, dec, x, n, normal);
+    else
+        hipLaunchKernelGGL(normals_fp32_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, dec, x, n, normal);
+    return check_launch("normals_fp32_kernel");
+}
+
+size_t nglod_trace_dense_workspace_size(void) { return 256; }
+
+int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const float* ray_d, int64_t n, int32_t lod,
+                      const nglod_trace_params_t* params, int32_t form, float* x, float* t, uint8_t* hit,
+                      float* normal, void* workspace, size_t workspace_bytes, uint64_t* evals_out,
+                      int32_t* ray_steps, void* stream) {
+    ModelArgs m;
+    int rc = make_model_args(model, lod, form, &m);
+    if (rc) return rc;
+    if (!params) return fail(NGLOD_ERR_ARG, "params is NULL");
+    if (params->num_steps < 1) return fail(NGLOD_ERR_ARG, "num_steps must be >= 1");
+    if (n < 0 || n > (int64_t)0x7fff0000) return fail(NGLOD_ERR_ARG, "n out of range");
+    if (n == 0) return NGLOD_OK;
+    if (!ray_o || !ray_d || !x || !t || !hit || !normal) return fail(NGLOD_ERR_ARG, "NULL pointer");
+    if (!workspace || workspace_bytes < nglod_trace_dense_workspace_size())
+        return fail(NGLOD_ERR_WORKSPACE, "workspace too small");
+    hipStream_t s = (hipStream_t)stream;
+    TraceArgs a;
+    a.m = m;
+    a.lod = lod;
+    a.dec = decoder_of(model, lod);
+    a.num_steps = params->num_steps;
+    a.normals = params->normals ? 1 : 0;
+    a.min_dis = (float)params->min_dis;
+    a.min_dis3 = (float)(params->min_dis * 3.0);
+    a.step = (float)params->step_size;
+    a.far_clip = (float)params->far_clip;
+    a.ray_o = ray_o;
+    a.ray_d = ray_d;
+    a.x = x;
+    a.t = t;
+    a.hit = hit;
+    a.normal = normal;
+    a.queue = (uint32_t*)workspace;
+    a.evals = (unsigned long long*)evals_out;
+    a.ray_steps = ray_steps;
+    split_queues(n, a.qbeg, a.qend);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, a.queue, kQueues, 0u);
+    rc = check_launch("fill_u32_kernel");
+    if (rc) return rc;
+    if (form == NGLOD_MIXED) return launch_trace_mixed(a, params->refine_steps, s);
+    if (m.pos_inv) {
+        const int grid = persistent_grid(trace_dense_fp32_kernel<true>, n, 256);
+        hipLaunchKernelGGL(trace_dense_fp32_kernel<true>, dim3(grid), dim3(256), 0, s, a);
+    } else {
+        const int grid = persistent_grid(trace_dense_fp32_kernel<false>, n, 256);
+        hipLaunchKernelGGL(trace_dense_fp32_kernel<false>, dim3(grid), dim3(256), 0, s, a);
+    }
+    return check_launch("trace_dense_fp32_kernel");
+}
+
+}  // extern "C"

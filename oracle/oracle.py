@@ -1,0 +1,173 @@
+"""Python binding of the CPU oracle (liboracle.so, built from oracle/*.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product package never
+imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+MAX_LODS = 16
+
+_f = ctypes.c_void_p
+
+
+class OrcModel(ctypes.Structure):
+    _fields_ = [
+        ("num_lods", ctypes.c_int32),
+        ("fdim", ctypes.c_int32),
+        ("hidden", ctypes.c_int32),
+        ("pos_invariant", ctypes.c_int32),
+        ("num_decoders", ctypes.c_int32),
+        ("res", ctypes.c_int32 * MAX_LODS),
+        ("grid", _f * MAX_LODS),
+        ("w0", _f * MAX_LODS),
+        ("b0", _f * MAX_LODS),
+        ("w1", _f * MAX_LODS),
+        ("b1", _f * MAX_LODS),
+    ]
+
+
+class OrcTraceParams(ctypes.Structure):
+    _fields_ = [
+        ("num_steps", ctypes.c_int32),
+        ("min_dis", ctypes.c_double),
+        ("step_size", ctypes.c_double),
+        ("far_clip", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class ModelArrays:
+    """Host copy of an OctreeSDF: corner-major FP32 grids + decoder weights."""
+
+    def __init__(self, grids, res, w0, b0, w1, b1, pos_invariant):
+        self.grids = [np.ascontiguousarray(g, dtype=np.float32).reshape(-1) for g in grids]
+        self.res = list(res)
+        self.w0 = [np.ascontiguousarray(w, dtype=np.float32) for w in w0]
+        self.b0 = [np.ascontiguousarray(b, dtype=np.float32) for b in b0]
+        self.w1 = [np.ascontiguousarray(w, dtype=np.float32).reshape(-1) for w in w1]
+        self.b1 = [np.ascontiguousarray(b, dtype=np.float32).reshape(-1) for b in b1]
+        self.pos_invariant = bool(pos_invariant)
+        self.fdim = self.grids[0].size // (self.res[0] + 1) ** 3
+        self.hidden = self.w0[0].shape[0]
+        self._s = None
+
+    @classmethod
+    def from_module(cls, net):
+        """From an OctreeSDF-like module (reference parameter names)."""
+        grids, res = [], []
+        for f in net.features:
+            fm = f.fm.detach().cpu().float()  # [1, C, D, H, W]
+            grids.append(fm[0].permute(1, 2, 3, 0).contiguous().numpy())
+            res.append(int(fm.shape[-1]) - 1)
+        w0 = [l[0].weight.detach().cpu().numpy() for l in net.louts]
+        b0 = [l[0].bias.detach().cpu().numpy() for l in net.louts]
+        w1 = [l[2].weight.detach().cpu().numpy() for l in net.louts]
+        b1 = [l[2].bias.detach().cpu().numpy() for l in net.louts]
+        return cls(grids, res, w0, b0, w1, b1, net.pos_invariant)
+
+    def struct(self):
+        if self._s is None:
+            s = OrcModel()
+            s.num_lods = len(self.grids)
+            s.fdim = self.fdim
+            s.hidden = self.hidden
+            s.pos_invariant = 1 if self.pos_invariant else 0
+            s.num_decoders = len(self.w0)
+            for i, g in enumerate(self.grids):
+                s.res[i] = self.res[i]
+                s.grid[i] = g.ctypes.data
+            for i in range(len(self.w0)):
+                s.w0[i] = self.w0[i].ctypes.data
+                s.b0[i] = self.b0[i].ctypes.data
+                s.w1[i] = self.w1[i].ctypes.data
+                s.b1[i] = self.b1[i].ctypes.data
+            self._s = s
+        return self._s
+
+
+def _dt(prec):
+    return np.float32 if prec == "f32" else np.float64
+
+
+def sdf(model, x, lod, prec="f32"):
+    x = np.ascontiguousarray(x, dtype=_dt(prec)).reshape(-1, 3)
+    out = np.empty(x.shape[0], dtype=_dt(prec))
+    fn = getattr(lib(), f"orc_sdf_{prec}")
+    fn(ctypes.byref(model.struct()), _p(x), ctypes.c_int64(x.shape[0]), ctypes.c_int(lod), _p(out))
+    return out
+
+
+def sdf_all(model, x, lod, prec="f32"):
+    x = np.ascontiguousarray(x, dtype=_dt(prec)).reshape(-1, 3)
+    out = np.empty((x.shape[0], lod + 1), dtype=_dt(prec))
+    fn = getattr(lib(), f"orc_sdf_all_{prec}")
+    fn(ctypes.byref(model.struct()), _p(x), ctypes.c_int64(x.shape[0]), ctypes.c_int(lod), _p(out))
+    return out
+
+
+def aabb(o, d, prec="f32"):
+    o = np.ascontiguousarray(o, dtype=_dt(prec)).reshape(-1, 3)
+    d = np.ascontiguousarray(d, dtype=_dt(prec)).reshape(-1, 3)
+    n = o.shape[0]
+    x = np.empty((n, 3), dtype=_dt(prec))
+    t = np.empty(n, dtype=_dt(prec))
+    hit = np.empty(n, dtype=np.uint8)
+    getattr(lib(), f"orc_aabb_{prec}")(_p(o), _p(d), _p(x), _p(t), _p(hit), ctypes.c_int64(n))
+    return x, t, hit.astype(bool)
+
+
+def trace(model, o, d, lod, num_steps=128, min_dis=3e-4, step_size=1.0, far_clip=10.0, prec="f32"):
+    """SphereTracer.forward per ray; returns dict x, t, hit, normal, evals
+    (all SDF evaluations), steps (in-loop evaluations) and margin (closest
+    approach of any termination input to its threshold)."""
+    o = np.ascontiguousarray(o, dtype=_dt(prec)).reshape(-1, 3)
+    d = np.ascontiguousarray(d, dtype=_dt(prec)).reshape(-1, 3)
+    n = o.shape[0]
+    P = OrcTraceParams(num_steps, min_dis, step_size, far_clip)
+    x = np.empty((n, 3), dtype=_dt(prec))
+    t = np.empty(n, dtype=_dt(prec))
+    hit = np.empty(n, dtype=np.uint8)
+    nrm = np.empty((n, 3), dtype=_dt(prec))
+    evals = np.empty(n, dtype=np.int32)
+    margin = np.empty(n, dtype=_dt(prec))
+    getattr(lib(), f"orc_trace_{prec}")(ctypes.byref(model.struct()), _p(o), _p(d), ctypes.c_int64(n),
+                                        ctypes.c_int(lod), ctypes.byref(P), _p(x), _p(t), _p(hit), _p(nrm),
+                                        _p(evals), _p(margin))
+    hit = hit.astype(bool)
+    _, _, entered = aabb(o, d, prec)
+    steps = evals - entered.astype(np.int32) - 6 * hit.astype(np.int32)
+    return dict(x=x, t=t, hit=hit, normal=nrm, evals=evals, margin=margin, steps=steps)
+
+
+def normals(model, x, lod, prec="f32"):
+    x = np.ascontiguousarray(x, dtype=_dt(prec)).reshape(-1, 3)
+    out = np.empty_like(x)
+    getattr(lib(), f"orc_normals_{prec}")(ctypes.byref(model.struct()), _p(x), ctypes.c_int64(x.shape[0]),
+                                          ctypes.c_int(lod), _p(out))
+    return out

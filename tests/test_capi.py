@@ -1,0 +1,68 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares;
+host-side argument checks return their status without touching a GPU."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from nglod_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(nglod_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    assert "nglod_trace_dense" in names and "nglod_sdf_fwd" in names and "nglod_aabb" in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_every_symbol():
+    assert set(declared_symbols()) <= set(_lib.SIGNATURES)
+
+
+def test_abi_version():
+    assert _lib.load().nglod_abi_version() == 1
+
+
+def test_host_argument_checks():
+    lib = _lib.load()
+    assert lib.nglod_aabb(None, None, -1, None, None, None, None) == _lib.ERR_ARG
+    assert lib.nglod_aabb(None, None, 0, None, None, None, None) == _lib.OK  # empty input
+    assert lib.nglod_aabb(None, None, 5, None, None, None, None) == _lib.ERR_ARG
+    assert b"NULL" in lib.nglod_last_error()
+    m = _lib.NglodModel()
+    m.num_lods = 5
+    m.feature_dim = 16
+    m.hidden_dim = 128
+    m.num_decoders = 5
+    assert lib.nglod_sdf_fwd(ctypes.byref(m), None, 10, 4, 0, None, None) == _lib.ERR_UNSUPPORTED
+    m.feature_dim = 32
+    assert lib.nglod_sdf_fwd(ctypes.byref(m), None, 10, 7, 0, None, None) == _lib.ERR_ARG  # lod out of range
+    p = _lib.NglodTraceParams()
+    p.num_steps = 0
+    for i in range(5):
+        m.res[i] = 4 << i
+        m.grid[i] = 4096 * (i + 1)
+        m.w0[i] = m.b0[i] = m.w1[i] = m.b1[i] = 4096
+    assert lib.nglod_trace_dense(ctypes.byref(m), None, None, 10, 4, ctypes.byref(p), 0, None, None, None, None,
+                                 None, 0, None, None, None) == _lib.ERR_ARG
+    p.num_steps = 8
+    assert lib.nglod_trace_dense(ctypes.byref(m), 16, 16, 10, 4, ctypes.byref(p), 0, 16, 16, 16, 16,
+                                 None, 0, None, None, None) == _lib.ERR_WORKSPACE
