@@ -44,27 +44,48 @@ def sphere_field(res, radius):
     return (torch.sqrt(x * x + y * y + z * z) - radius).float()
 
 
-def make_sphere_model(radius=0.6, num_lods=5, base_lod=2, pos_invariant=True, seed=0, device="cuda",
+def _telescoped_levels(field_fn, resolutions):
+    """Per-level corner values whose level sums telescope: level i holds
+    F - I_{i-1}(F) at its corners, where I_{i-1} is the trilinear interpolant
+    of F on level i-1.  A trilinear function on a coarse cell is trilinear on
+    every finer sub-cell, so sum_{i<=L} interp_i(level_i) = I_L(F): the model
+    at lod L is the level-L interpolant of F."""
+    import torch.nn.functional as F
+
+    levels, prev = [], None
+    for r in resolutions:
+        full = field_fn(r).double()  # [z, y, x]
+        if prev is None:
+            levels.append(full)
+        else:
+            c = torch.linspace(-1.0, 1.0, r + 1, dtype=torch.float64)
+            z, y, x = torch.meshgrid(c, c, c, indexing="ij")
+            grid = torch.stack([x, y, z], -1).reshape(1, -1, 1, 1, 3)
+            coarse = F.grid_sample(prev[None, None], grid, align_corners=True, padding_mode="border")
+            levels.append(full - coarse.reshape(r + 1, r + 1, r + 1))
+        prev = full
+    return [l.float() for l in levels]
+
+
+def make_sphere_model(radius=0.6, num_lods=5, base_lod=2, pos_invariant=True, seed=0, device="cpu",
                       noise=1e-3):
-    """A model whose SDF is (up to `noise`-scale perturbations) the trilinear
-    interpolant of a sphere's SDF on the finest grid: channel 0 of the finest
-    volume holds the sphere field, channel 0 of the other volumes is zero, and
+    """A model whose SDF at lod L is, up to `noise`-scale perturbations, the
+    trilinear interpolant of a sphere's SDF on the level-L grid.  Channel 0 of
+    the volumes holds telescoped sphere values (see _telescoped_levels) and
     decoder row 0 passes channel 0 through the ReLU with a +2 offset.  The
-    remaining channels and rows keep small seeded random values, so every
-    weight still takes part in the arithmetic."""
+    other channels and rows keep small seeded random values, so every weight
+    still takes part in the arithmetic."""
     net = make_model(num_lods=num_lods, base_lod=base_lod, pos_invariant=pos_invariant, seed=seed, device="cpu")
     g = torch.Generator().manual_seed(seed + 1000)
     off = 0 if pos_invariant else 3
+    levels = _telescoped_levels(lambda r: sphere_field(r, radius), [f.fsize for f in net.features])
     with torch.no_grad():
-        for i, f in enumerate(net.features):
-            f.fm[0, 0].zero_()
-            if i == num_lods - 1:
-                f.fm[0, 0].copy_(sphere_field(f.fsize, radius))
+        for f, lv in zip(net.features, levels):
+            f.fm[0, 0].copy_(lv)
         for l in net.louts:
             w0 = l[0].weight
             w0.mul_(noise)
             w0[0].zero_()
-            w0[0, off] = 1.0
             w0[:, off].zero_()
             w0[0, off] = 1.0
             l[0].bias.copy_(torch.rand(l[0].bias.shape, generator=g) * noise)

@@ -34,10 +34,13 @@ def rand_rays(n, seed, dist=(1.5, 4.0)):
     return o.astype(np.float32), d.astype(np.float32)
 
 
-def ambiguous_rays(orc, other_steps, num_steps, margin=2e-6):
+def ambiguous_rays(orc, other_steps, num_steps, margin=2e-6, cap_is_ambiguous=True):
     """Threshold-ambiguous rays of a trace comparison: a termination input of
     the oracle within `margin` of its threshold, a different number of steps
     in the two implementations (a flipped termination decision), or the step
     cap reached (an unconverged, grazing trajectory whose end point is not a
     surface point)."""
-    return (orc["margin"] < margin) | (orc["steps"] != other_steps) | (orc["steps"] >= num_steps)
+    amb = (orc["margin"] < margin) | (orc["steps"] != other_steps)
+    if cap_is_ambiguous:
+        amb |= orc["steps"] >= num_steps
+    return amb
