@@ -9,7 +9,6 @@ measured on one GPU.
 """
 import argparse
 import json
-import math
 import os
 import platform
 import sys
@@ -73,7 +72,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--form", choices=["fp32", "mixed"], default="fp32")
+    ap.add_argument("--form", choices=["fp32", "mixed", "both"], default="both")
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
     ap.add_argument("--lod", type=int, default=4)
@@ -94,73 +93,78 @@ def main():
     net.lod = args.lod
     o, d = workloads.camera_rays(args.width, args.height, eye=(0.0, 0.0, 3.0), device=dev)
     n = o.shape[0]
-    tracer = SphereTracer(num_steps=args.num_steps, min_dis=3e-4, camera_clamp=[-5, 10], step_size=1.0,
-                          form=args.form)
 
-    for _ in range(args.warmup):
-        tracer(net, o, d)
-    torch.cuda.synchronize()
+    def run_form(form):
+        tracer = SphereTracer(num_steps=args.num_steps, min_dis=3e-4, camera_clamp=[-5, 10], step_size=1.0,
+                              form=form)
+        for _ in range(args.warmup):
+            tracer(net, o, d)
+        torch.cuda.synchronize()
+        # Timed region: K frames back to back; HIP events on the stream the
+        # kernels are launched on (torch's current stream) bracket each frame.
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            starts[k].record()
+            rb = tracer(net, o, d)
+            ends[k].record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        kernel_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+        ev = tracer.last_evals.cpu().tolist()  # [FP32-form evals, mixed-form evals] of the last frame
+        alg = ev[0] * bytes_per_query(args.lod, elem=4) + ev[1] * bytes_per_query(args.lod, elem=2) + n * (24 + 29)
+        return {
+            "rays_per_s": n * args.steps / wall,
+            "ms_per_step": wall * 1e3 / args.steps,
+            "kernel_ms": kernel_ms,
+            "sdf_evals_fp32": int(ev[0]),
+            "sdf_evals_mixed": int(ev[1]),
+            "hits": int(rb.hit.sum().item()),
+            "algorithmic_bytes": int(alg),
+            "achieved_GBs": alg / (kernel_ms * 1e-3) / 1e9,
+        }
 
-    # Timed region: K frames back to back.
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    evals = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    outs = []
-    for k in range(args.steps):
-        starts[k].record()
-        rb = tracer(net, o, d)
-        ends[k].record()
-        outs.append(tracer.last_evals)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
-    evals = int(outs[-1].item())
-    hits = int(rb.hit.sum().item())
-
-    ms_per_step = wall * 1e3 / args.steps
-    rays_per_s = n * args.steps / wall
-    elem = 2 if args.form == "mixed" else 4
-    bq = bytes_per_query(args.lod, elem=elem)
-    alg_bytes = evals * bq + n * (24 + 29)
-    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
-    kname = "trace_dense_fp32_kernel" if args.form == "fp32" else "trace_dense_mixed_kernel"
-    tr = load_traffic(f"{kname}_{args.width}x{args.height}_lod{args.lod}.traffic.json")
+    forms = ["fp32", "mixed"] if args.form == "both" else [args.form]
+    res = {f: run_form(f) for f in forms}
+    primary = "mixed" if "mixed" in res else "fp32"
+    r = res[primary]
+    kname = "trace_dense_fp32_kernel" if primary == "fp32" else "trace_mixed_pass1_kernel+trace_mixed_pass2_kernel"
+    tr = load_traffic(f"{primary}_{args.width}x{args.height}_lod{args.lod}.traffic.json")
 
     line = {
         "metric": "traced rays/s @1280x720 LOD4",
-        "value": rays_per_s,
+        "value": r["rays_per_s"],
         "unit": "rays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.form == "fp32" else "f16-grid/bf16-mfma",
+        "dtype": "f32" if primary == "fp32" else "bf16",
         "data": "synthetic",
         "config": {
             "workload": f"sphere trace {args.width}x{args.height} pinhole frame, LOD {args.lod}, "
                         f"max {args.num_steps} steps, min_dis 3e-4, FD normals; {workloads.BENCH_MODEL_NAME}",
             "rays": n,
             "lod": args.lod,
-            "form": args.form,
-            "sdf_evals_per_frame": evals,
-            "hits": hits,
+            "form": primary,
+            "forms": res,
         },
         "roofline": {
             "bound": "hbm",
-            "achieved": achieved,
+            "achieved": r["achieved_GBs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "frac": r["achieved_GBs"] / HBM_PEAK_GBS,
             "traffic": (tr or {}).get("bytes_per_launch"),
             "kernel": kname,
-            "kernel_ms": kernel_ms,
-            "bytes_per_query": bq,
-            "algorithmic_bytes": alg_bytes,
+            "kernel_ms": r["kernel_ms"],
+            "bytes_per_query": {"fp32": bytes_per_query(args.lod, elem=4), "mixed": bytes_per_query(args.lod, elem=2)},
+            "algorithmic_bytes": r["algorithmic_bytes"],
         },
     }
     if args.cpu_sample > 0:

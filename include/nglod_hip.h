@@ -67,7 +67,13 @@ typedef struct nglod_trace_params {
     double step_size;   /* scales every in-loop SDF step */
     double far_clip;    /* camera_clamp[1] */
     int32_t normals;    /* 1: central-difference normals at hit points */
-    int32_t refine_steps; /* NGLOD_MIXED only: FP32 steps re-evaluated at the end */
+    int32_t refine_steps; /* NGLOD_MIXED only (1..4): when the mixed-precision
+                             march stops on a surface test, its last
+                             refine_steps evaluations are re-evaluated in FP32
+                             and the march goes on in FP32 to its FP32 stop */
+    double return_dist;   /* NGLOD_MIXED only: an FP32 step that grows past this
+                             (the ray leaves the surface: a near miss) returns
+                             the ray to mixed precision, at most twice */
 } nglod_trace_params_t;
 
 int nglod_abi_version(void);
@@ -103,8 +109,9 @@ int nglod_sdf_fwd(const nglod_model_t* model, const float* x, int64_t n, int32_t
 int nglod_sdf_fwd_all(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
                       float* out, void* stream);
 
-/* Workspace bytes nglod_trace_dense needs (ray queue counters + statistics). */
-size_t nglod_trace_dense_workspace_size(void);
+/* Workspace bytes nglod_trace_dense needs for n rays in `form` (ray queue
+ * counters; for NGLOD_MIXED also the FP32 hand-off list, 48 B per ray). */
+size_t nglod_trace_dense_workspace_size(int64_t n, int32_t form);
 
 /* Persistent sphere tracer with SphereTracer semantics
  * (sdf-net/lib/tracer/SphereTracer.py:41-132 incl. box entry :53, FD normals
@@ -113,7 +120,11 @@ size_t nglod_trace_dense_workspace_size(void);
  * position of the last evaluation, which lags t by one step as in the
  * reference), t [n], hit [n], normal [n][3] (zeros for misses).  Per-ray
  * arithmetic does not depend on which other rays are in the launch.
- * evals_out (device, may be NULL): total SDF evaluations, uint64.
+ * NGLOD_MIXED: a mixed-precision persistent march, then an FP32 persistent
+ * pass over the rays handed off at their surface stop (see refine_steps),
+ * then FP32 normals of the hits.
+ * evals_out (device, may be NULL): uint64[2], SDF evaluations in FP32 form
+ * ([0]) and in mixed form ([1]), accumulated (the caller zeroes it).
  * ray_steps (device, may be NULL): per ray, the in-loop SDF evaluations
  * (the reference's loop trips in which the ray was active), int32 [n]. */
 int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const float* ray_d,
@@ -123,14 +134,15 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
                       int32_t* ray_steps, void* stream);
 
 /* Central-difference normals h = 1/192 then normalize(eps=1e-5)
- * (sdf-net/lib/diffutils.py:61-70, SphereTracer.py:128-130) at points x [n][3]. */
+ * (sdf-net/lib/diffutils.py:61-70, SphereTracer.py:128-130) at points x [n][3].
+ * Computed in FP32 for either form. */
 int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
                      int32_t form, float* normal, void* stream);
 
-/* Converts an FP32 model's grids to FP16 and W0 to the BF16 MFMA fragment
- * layout used by NGLOD_MIXED.  grid_h[i] must hold (res[i]+1)^3*feature_dim
- * uint16, w0_bf16[i] hidden*32 uint16 (pos_invariant) or hidden*64 (xyz
- * input).  Replaces the half-precision export of SOL_NGLOD
+/* Converts an FP32 model's grids to FP16 and the feature columns of W0 to the
+ * BF16 MFMA fragment layout used by NGLOD_MIXED.  grid_h[i] must hold
+ * (res[i]+1)^3*feature_dim uint16, w0_bf16[i] hidden*feature_dim uint16 (the
+ * xyz columns of a position-dependent decoder stay FP32).  Replaces the half-precision export of SOL_NGLOD
  * (sdf-net/lib/models/SOL_NGLOD.py:33-46), kept on device. */
 int nglod_pack_mixed(const nglod_model_t* model, void* stream);
 

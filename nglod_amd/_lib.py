@@ -67,6 +67,7 @@ class NglodTraceParams(ctypes.Structure):
         ("far_clip", ctypes.c_double),
         ("normals", _i32),
         ("refine_steps", _i32),
+        ("return_dist", ctypes.c_double),
     ]
 
 
@@ -78,7 +79,7 @@ SIGNATURES = {
     "nglod_grid_sample": (ctypes.c_int, [_p, _i32, _i32, _p, _i64, _p, _p]),
     "nglod_sdf_fwd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _p, _p]),
     "nglod_sdf_fwd_all": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _p, _p]),
-    "nglod_trace_dense_workspace_size": (_sz, []),
+    "nglod_trace_dense_workspace_size": (_sz, [_i64, _i32]),
     "nglod_trace_dense": (
         ctypes.c_int,
         [ctypes.POINTER(NglodModel), _p, _p, _i64, _i32, ctypes.POINTER(NglodTraceParams), _i32,

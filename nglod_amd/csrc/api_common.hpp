@@ -50,7 +50,8 @@ inline int make_model_args(const nglod_model_t* m, int lod, int form, ModelArgs*
         a->grid[i] = m->grid[i];
         a->grid_h[i] = m->grid_h[i];
         if (i <= lod) {
-            if (form == NGLOD_FP32 && (!m->grid[i] || !aligned16(m->grid[i])))
+            // the mixed form re-evaluates near the surface in FP32: it needs both grids
+            if (!m->grid[i] || !aligned16(m->grid[i]))
                 return fail(NGLOD_ERR_ARG, "grid pointer NULL or not 16-byte aligned");
             if (form == NGLOD_MIXED && (!m->grid_h[i] || !aligned16(m->grid_h[i])))
                 return fail(NGLOD_ERR_ARG, "fp16 grid pointer NULL or not 16-byte aligned");
@@ -80,7 +81,6 @@ int device_cu_count();
 struct TraceArgs;
 // Mixed-precision launchers (mixed.hip).
 int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out, hipStream_t s);
-int launch_normals_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* nrm, hipStream_t s);
-int launch_trace_mixed(const TraceArgs& a, int refine_steps, hipStream_t s);
+int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s);
 
 }  // namespace nglod

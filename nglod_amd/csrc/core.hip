@@ -266,14 +266,16 @@ int nglod_sdf_fwd_all(const nglod_model_t* model, const float* x, int64_t n, int
 
 int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod, int32_t form, float* normal,
                      void* stream) {
+    // Normals are FP32 in both forms (finite differences of h = 1/192 need
+    // more than BF16 resolution); `form` only has to name a known form.
+    if (form != NGLOD_FP32 && form != NGLOD_MIXED) return fail(NGLOD_ERR_ARG, "unknown form");
     ModelArgs m;
-    int rc = make_model_args(model, lod, form, &m);
+    int rc = make_model_args(model, lod, NGLOD_FP32, &m);
     if (rc) return rc;
     if (n < 0) return fail(NGLOD_ERR_ARG, "n < 0");
     if (n == 0) return NGLOD_OK;
     if (!x || !normal) return fail(NGLOD_ERR_ARG, "NULL pointer");
     const int dec = decoder_of(model, lod);
-    if (form == NGLOD_MIXED) return launch_normals_mixed(m, lod, dec, x, n, normal, (hipStream_t)stream);
     const int64_t blocks = (n + 255) / 256;
     if (m.pos_inv)
         hipLaunchKernelGGL(normals_fp32_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, dec, x, n, normal);
@@ -282,7 +284,12 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
     return check_launch("normals_fp32_kernel");
 }
 
-size_t nglod_trace_dense_workspace_size(void) { return 256; }
+static constexpr size_t kWsCounters = 256;  // queue[8], handoff_count/queue, ncount, ret_count/queue
+
+size_t nglod_trace_dense_workspace_size(int64_t n, int32_t form) {
+    if (n < 0) n = 0;
+    return kWsCounters + (form == NGLOD_MIXED ? (size_t)n * (2 * sizeof(Handoff) + sizeof(NormalItem)) : 0);
+}
 
 int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const float* ray_d, int64_t n, int32_t lod,
                       const nglod_trace_params_t* params, int32_t form, float* x, float* t, uint8_t* hit,
@@ -293,11 +300,14 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     if (rc) return rc;
     if (!params) return fail(NGLOD_ERR_ARG, "params is NULL");
     if (params->num_steps < 1) return fail(NGLOD_ERR_ARG, "num_steps must be >= 1");
+    if (form == NGLOD_MIXED && (params->refine_steps < 1 || params->refine_steps > 4))
+        return fail(NGLOD_ERR_ARG, "refine_steps must be 1..4");
     if (n < 0 || n > (int64_t)0x7fff0000) return fail(NGLOD_ERR_ARG, "n out of range");
     if (n == 0) return NGLOD_OK;
     if (!ray_o || !ray_d || !x || !t || !hit || !normal) return fail(NGLOD_ERR_ARG, "NULL pointer");
-    if (!workspace || workspace_bytes < nglod_trace_dense_workspace_size())
+    if (!workspace || workspace_bytes < nglod_trace_dense_workspace_size(n, form))
         return fail(NGLOD_ERR_WORKSPACE, "workspace too small");
+    if (!aligned16(workspace)) return fail(NGLOD_ERR_ARG, "workspace not 16-byte aligned");
     hipStream_t s = (hipStream_t)stream;
     TraceArgs a;
     a.m = m;
@@ -316,13 +326,27 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     a.hit = hit;
     a.normal = normal;
     a.queue = (uint32_t*)workspace;
+    a.handoff_count = a.queue + kQueues;
+    a.handoff_queue = a.queue + kQueues + 1;
+    a.ncount = a.queue + kQueues + 2;
+    a.handoff = (Handoff*)((char*)workspace + kWsCounters);
+    a.ret_count = a.queue + kQueues + 3;
+    a.ret_queue = a.queue + kQueues + 4;
+    a.ret = (Handoff*)((char*)workspace + kWsCounters + (size_t)n * sizeof(Handoff));
+    a.nlist = (NormalItem*)((char*)workspace + kWsCounters + 2 * (size_t)n * sizeof(Handoff));
+    a.from_returns = 0;
+    a.allow_return = 0;
+    a.return_dist = (float)params->return_dist;
+    a.defer_normals = form == NGLOD_MIXED ? 1 : 0;
+    a.replay = 0;
+    a.replay_steps = params->refine_steps < 1 ? 1 : params->refine_steps;
     a.evals = (unsigned long long*)evals_out;
     a.ray_steps = ray_steps;
     split_queues(n, a.qbeg, a.qend);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, a.queue, kQueues, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, a.queue, kQueues + 5, 0u);
     rc = check_launch("fill_u32_kernel");
     if (rc) return rc;
-    if (form == NGLOD_MIXED) return launch_trace_mixed(a, params->refine_steps, s);
+    if (form == NGLOD_MIXED) return launch_trace_mixed(a, n, s);
     if (m.pos_inv) {
         const int grid = persistent_grid(trace_dense_fp32_kernel<true>, n, 256);
         hipLaunchKernelGGL(trace_dense_fp32_kernel<true>, dim3(grid), dim3(256), 0, s, a);

@@ -1,0 +1,189 @@
+// Mixed-precision SDF evaluation for one wavefront (64 queries, one per lane):
+// FP16 corner-major grids gathered with packed FP16 FMAs, then the first
+// decoder layer as BF16 MFMA 32x32x16 tiles with FP32 accumulation, the
+// second layer (128 -> 1) and the ReLU in FP32 on the MFMA output registers.
+//
+// MFMA operand layout (v_mfma_f32_32x32x16_bf16): lane l holds A[m = l%32][k in
+// half l/32] and B[k in half l/32][n = l%32]; A and B use the same (half,
+// element) -> k map, so any consistent channel order is valid.  Output: lane l
+// holds C[m][n = l%32] for m = 8*(i/4) + 4*(l/32) + i%4, i = 0..15.
+//
+// A = W0 (hidden x 32 feature channels), B = features^T (32 x 32 queries).
+// The 64 queries of a wave form two 32-query tiles; a lane's own features
+// feed tile (l/32) through one lane-half swap.
+#pragma once
+
+#include <hip/hip_fp16.h>
+
+#include "device_common.hpp"
+
+namespace nglod {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+
+// Per-workgroup constants of the mixed decoder, staged once in LDS (11 KB):
+// the W0 A fragments ([h tile 0..3][k block 0..1][lane], 8 bf16 each), b0, w1
+// and the FP32 xyz columns of W0.
+struct MixedLds {
+    uint4 wfrag[4 * 2 * 64];
+    float b0[kH];
+    float w1[kH];
+    float wxyz[kH][4];  // W0 columns 0..2 (xyz input), FP32; [3] unused
+};
+
+// Stage decoder `dec` into LDS.  Every thread of the block must call this,
+// followed by __syncthreads().
+__device__ __forceinline__ void stage_mixed_lds(const ModelArgs& m, int dec, MixedLds* s) {
+    const int din = m.pos_inv ? kC : kC + 3;
+    const uint4* frag = reinterpret_cast<const uint4*>(m.w0_bf16[dec]);
+    for (int j = threadIdx.x; j < 4 * 2 * 64; j += blockDim.x) s->wfrag[j] = frag[j];
+    for (int j = threadIdx.x; j < kH; j += blockDim.x) {
+        s->b0[j] = m.b0[dec][j];
+        s->w1[j] = m.w1[dec][j];
+        if (!m.pos_inv) {
+            s->wxyz[j][0] = m.w0[dec][j * din + 0];
+            s->wxyz[j][1] = m.w0[dec][j * din + 1];
+            s->wxyz[j][2] = m.w0[dec][j * din + 2];
+        } else {
+            s->wxyz[j][0] = s->wxyz[j][1] = s->wxyz[j][2] = 0.0f;
+        }
+        s->wxyz[j][3] = 0.0f;
+    }
+}
+
+// Accumulates w * grid_h[corner][0..31] into acc (16 packed FP16 pairs).
+__device__ __forceinline__ void corner_hfma(__half2 (&acc)[kC / 2], const __half* __restrict__ p, __half2 w) {
+    const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+    for (int v = 0; v < kC / 8; ++v) {
+        const uint4 g = q[v];
+        acc[4 * v + 0] = __hfma2(__builtin_bit_cast(__half2, g.x), w, acc[4 * v + 0]);
+        acc[4 * v + 1] = __hfma2(__builtin_bit_cast(__half2, g.y), w, acc[4 * v + 1]);
+        acc[4 * v + 2] = __hfma2(__builtin_bit_cast(__half2, g.z), w, acc[4 * v + 2]);
+        acc[4 * v + 3] = __hfma2(__builtin_bit_cast(__half2, g.w), w, acc[4 * v + 3]);
+    }
+}
+
+__device__ __forceinline__ void gather_fp16(const ModelArgs& m, int lod, float px, float py, float pz,
+                                            __half2 (&acc)[kC / 2]) {
+#pragma unroll
+    for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
+    for (int l = 0; l <= lod; ++l) {
+        const int r = m.res[l];
+        const int S = r + 1;
+        const AxisW ax = axis_weights(px, r);
+        const AxisW ay = axis_weights(py, r);
+        const AxisW az = axis_weights(pz, r);
+        const __half* base = reinterpret_cast<const __half*>(m.grid_h[l]) +
+                             ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
+        const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
+        const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
+        const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
+        corner_hfma(acc, base, __float2half2_rn(wxy00 * az.w0));
+        corner_hfma(acc, base + sx, __float2half2_rn(wxy10 * az.w0));
+        corner_hfma(acc, base + sy, __float2half2_rn(wxy01 * az.w0));
+        corner_hfma(acc, base + sy + sx, __float2half2_rn(wxy11 * az.w0));
+        corner_hfma(acc, base + sz, __float2half2_rn(wxy00 * az.w1));
+        corner_hfma(acc, base + sz + sx, __float2half2_rn(wxy10 * az.w1));
+        corner_hfma(acc, base + sz + sy, __float2half2_rn(wxy01 * az.w1));
+        corner_hfma(acc, base + sz + sy + sx, __float2half2_rn(wxy11 * az.w1));
+    }
+}
+
+__device__ __forceinline__ uint32_t pack_bf16(__half2 h) {
+    const float2 f = __half22float2(h);
+    const bf16x2v v = {(__bf16)f.x, (__bf16)f.y};
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+// Second layer over one 32x32 output tile held in registers: sum of
+// w1[row] * relu(C + b0[row] + wxyz[row] . xyz) over this lane's 16 rows.
+__device__ __forceinline__ float layer2_partial(const f32x16& c, int h, uint32_t half, const MixedLds* s, bool xyz,
+                                                float px, float py, float pz) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const int row0 = 32 * h + 8 * g + 4 * (int)half;
+        const float4 b = *reinterpret_cast<const float4*>(&s->b0[row0]);
+        const float4 w = *reinterpret_cast<const float4*>(&s->w1[row0]);
+        float v0 = c[4 * g + 0] + b.x, v1 = c[4 * g + 1] + b.y, v2 = c[4 * g + 2] + b.z, v3 = c[4 * g + 3] + b.w;
+        if (xyz) {
+            const float4 q0 = *reinterpret_cast<const float4*>(s->wxyz[row0 + 0]);
+            const float4 q1 = *reinterpret_cast<const float4*>(s->wxyz[row0 + 1]);
+            const float4 q2 = *reinterpret_cast<const float4*>(s->wxyz[row0 + 2]);
+            const float4 q3 = *reinterpret_cast<const float4*>(s->wxyz[row0 + 3]);
+            v0 = __builtin_fmaf(q0.x, px, __builtin_fmaf(q0.y, py, __builtin_fmaf(q0.z, pz, v0)));
+            v1 = __builtin_fmaf(q1.x, px, __builtin_fmaf(q1.y, py, __builtin_fmaf(q1.z, pz, v1)));
+            v2 = __builtin_fmaf(q2.x, px, __builtin_fmaf(q2.y, py, __builtin_fmaf(q2.z, pz, v2)));
+            v3 = __builtin_fmaf(q3.x, px, __builtin_fmaf(q3.y, py, __builtin_fmaf(q3.z, pz, v3)));
+        }
+        acc = __builtin_fmaf(w.x, fmaxf(v0, 0.0f), acc);
+        acc = __builtin_fmaf(w.y, fmaxf(v1, 0.0f), acc);
+        acc = __builtin_fmaf(w.z, fmaxf(v2, 0.0f), acc);
+        acc = __builtin_fmaf(w.w, fmaxf(v3, 0.0f), acc);
+    }
+    return acc;
+}
+
+// SDF of the 64 lanes' query points.  Every lane of the wave must call this
+// (no divergent branch around it): the MFMAs and lane swaps are wave-wide.
+// Lanes without a query pass any point in [-1,1]^3 and ignore the result.
+template <bool POSINV>
+__device__ __forceinline__ float sdf_mixed_wave(const ModelArgs& m, int lod, const MixedLds* s, float b1,
+                                                uint32_t lane, float px, float py, float pz) {
+    __half2 acc[kC / 2];
+    gather_fp16(m, lod, px, py, pz, acc);
+    const uint32_t half = lane >> 5;
+    const bool lo = half == 0;
+    // own channels 16kb + 0..7 (Y) and 16kb + 8..15 (X) as bf16
+    bf16x8 B0[2], B1[2];  // [kb] of tile 0 and tile 1
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        uint32_t Y[4], X[4], Ys[4], Xs[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            Y[j] = pack_bf16(acc[8 * kb + j]);
+            X[j] = pack_bf16(acc[8 * kb + 4 + j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            Ys[j] = __shfl_xor(Y[j], 32);
+            Xs[j] = __shfl_xor(X[j], 32);
+        }
+        uint4 t0, t1;
+        t0.x = lo ? Y[0] : Xs[0];
+        t0.y = lo ? Y[1] : Xs[1];
+        t0.z = lo ? Y[2] : Xs[2];
+        t0.w = lo ? Y[3] : Xs[3];
+        t1.x = lo ? Ys[0] : X[0];
+        t1.y = lo ? Ys[1] : X[1];
+        t1.z = lo ? Ys[2] : X[2];
+        t1.w = lo ? Ys[3] : X[3];
+        B0[kb] = __builtin_bit_cast(bf16x8, t0);
+        B1[kb] = __builtin_bit_cast(bf16x8, t1);
+    }
+    // query coordinates of each tile's column (tile t, column l%32 = lane l%32 + 32t)
+    const float qx0 = __shfl(px, lane & 31), qy0 = __shfl(py, lane & 31), qz0 = __shfl(pz, lane & 31);
+    const float qx1 = __shfl(px, (lane & 31) + 32), qy1 = __shfl(py, (lane & 31) + 32),
+                qz1 = __shfl(pz, (lane & 31) + 32);
+    float part0 = 0.0f, part1 = 0.0f;
+#pragma unroll 1
+    for (int h = 0; h < 4; ++h) {
+        const bf16x8 a0 = __builtin_bit_cast(bf16x8, s->wfrag[(h * 2 + 0) * 64 + lane]);
+        const bf16x8 a1 = __builtin_bit_cast(bf16x8, s->wfrag[(h * 2 + 1) * 64 + lane]);
+        f32x16 c0 = {}, c1 = {};
+        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B0[0], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B1[0], c1, 0, 0, 0);
+        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, B0[1], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, B1[1], c1, 0, 0, 0);
+        part0 += layer2_partial(c0, h, half, s, !POSINV, qx0, qy0, qz0);
+        part1 += layer2_partial(c1, h, half, s, !POSINV, qx1, qy1, qz1);
+    }
+    part0 += __shfl_xor(part0, 32);
+    part1 += __shfl_xor(part1, 32);
+    return (lo ? part0 : part1) + b1;
+}
+
+}  // namespace nglod

@@ -32,7 +32,31 @@ __device__ __forceinline__ void normalize_grad(float g0, float g1, float g2, flo
 // its own range moves on to the next ones.
 constexpr int kQueues = 8;
 
-enum : int { PH_IDLE = -1, PH_INIT = 0, PH_MARCH = 1, PH_N0 = 2, PH_N5 = 7 };
+// PH_NQ: a finished hit whose normal is computed by a separate pass (the ray's
+// x, t, hit are already written; it waits to be appended to the normal list).
+// PH_REPLAY: the mixed-precision march stopped on a surface test; the kernel
+// rewinds the last steps and hands them to the FP32 pass.
+enum : int { PH_IDLE = -1, PH_INIT = 0, PH_MARCH = 1, PH_N0 = 2, PH_N5 = 7, PH_NQ = 8, PH_REPLAY = 9 };
+
+// A finished hit waiting for its normal (mixed form).  16 bytes.
+struct NormalItem {
+    int32_t ray;
+    float x0, x1, x2;
+};
+
+// State of a ray handed from the mixed-precision march to the FP32 pass:
+// either a pending evaluation at x (phase INIT or MARCH, re-evaluated in
+// FP32) or the normal taps of a finished hit (phase N0).  48 bytes.
+struct Handoff {
+    int32_t ray;
+    int32_t phase;
+    int32_t it;
+    int32_t steps;
+    float t, D, Dp;
+    float x0, x1, x2;
+    int32_t hitflag;
+    int32_t pad;
+};
 
 struct TraceArgs {
     ModelArgs m;
@@ -45,8 +69,23 @@ struct TraceArgs {
     uint8_t* hit;
     float* normal;
     uint32_t* queue;
-    unsigned long long* evals;
+    unsigned long long* evals;  // [0] FP32-form evaluations, [1] mixed-form evaluations
     int32_t* ray_steps;
+    // Mixed form: rays handed from the mixed march to the FP32 pass.
+    int replay;        // 1 in the mixed march: a surface stop becomes PH_REPLAY
+    int replay_steps;  // evaluations rewound and re-evaluated in FP32
+    struct Handoff* handoff;
+    uint32_t* handoff_count;
+    uint32_t* handoff_queue;
+    struct Handoff* ret;     // rays handed back from FP32 to the mixed march
+    uint32_t* ret_count;
+    uint32_t* ret_queue;
+    int from_returns;        // pass 1 refills from `ret` instead of the ray queues
+    int allow_return;        // pass 2 may hand rays back (not in the last round)
+    float return_dist;       // FP32 step length above which a ray goes back to mixed
+    int defer_normals;  // 1: finished hits go to the normal list instead of in-lane taps
+    NormalItem* nlist;
+    uint32_t* ncount;
     int64_t qbeg[kQueues];
     int64_t qend[kQueues];
 };
@@ -82,7 +121,15 @@ struct RayState {
     __device__ __forceinline__ void finish(const TraceArgs& a) {
         const bool outside = (fabsf(x0) > 1.0f) || (fabsf(x1) > 1.0f) || (fabsf(x2) > 1.0f);
         const bool hit = hitflag && !outside;
-        if (hit && a.normals) {
+        if (hit && a.normals && a.defer_normals) {
+            a.x[3 * ray] = x0;
+            a.x[3 * ray + 1] = x1;
+            a.x[3 * ray + 2] = x2;
+            a.t[ray] = t;
+            a.hit[ray] = 1;
+            if (a.ray_steps) a.ray_steps[ray] = steps;
+            phase = PH_NQ;
+        } else if (hit && a.normals) {
             phase = PH_N0;
             qx = x0 + kFdH;
             qy = x1 + 0.0f;
@@ -101,6 +148,10 @@ struct RayState {
         hitflag = fabsf(t) < a.far_clip;
         const bool cond = (fabsf(D) > a.min_dis) && (fabsf((D + Dp) * 0.5f) > a.min_dis3) && hitflag;
         if (!cond) {
+            if (a.replay && hitflag) {
+                phase = PH_REPLAY;
+                return;
+            }
             finish(a);
             return;
         }
@@ -140,6 +191,68 @@ struct RayState {
             hitflag = (a.num_steps >= 1) && (fabsf(t) < a.far_clip);
             finish(a);
         }
+    }
+
+    // Pass 2 of the mixed form: continue a ray from its hand-off record.
+    __device__ __forceinline__ void resume(const TraceArgs& a, const Handoff& h) {
+        ray = h.ray;
+        ox = a.ray_o[3 * ray];
+        oy = a.ray_o[3 * ray + 1];
+        oz = a.ray_o[3 * ray + 2];
+        dx = a.ray_d[3 * ray];
+        dy = a.ray_d[3 * ray + 1];
+        dz = a.ray_d[3 * ray + 2];
+        phase = h.phase;
+        it = h.it;
+        steps = h.steps;
+        t = h.t;
+        D = h.D;
+        Dp = h.Dp;
+        x0 = h.x0;
+        x1 = h.x1;
+        x2 = h.x2;
+        hitflag = h.hitflag != 0;
+        if (phase == PH_N0) {
+            qx = x0 + kFdH;
+            qy = x1 + 0.0f;
+            qz = x2 + 0.0f;
+        } else {
+            qx = x0;
+            qy = x1;
+            qz = x2;
+        }
+    }
+
+    __device__ __forceinline__ void queue_normal(const TraceArgs& a, uint32_t slot) {
+        NormalItem it_;
+        it_.ray = (int32_t)ray;
+        it_.x0 = x0;
+        it_.x1 = x1;
+        it_.x2 = x2;
+        a.nlist[slot] = it_;
+        phase = PH_IDLE;
+        ray = -1;
+    }
+
+    __device__ __forceinline__ void hand_off(const TraceArgs& a, uint32_t slot) { hand_off_to(a.handoff, slot); }
+
+    __device__ __forceinline__ void hand_off_to(Handoff* list, uint32_t slot) {
+        Handoff h;
+        h.ray = (int32_t)ray;
+        h.phase = phase;
+        h.it = it;
+        h.steps = steps;
+        h.t = t;
+        h.D = D;
+        h.Dp = Dp;
+        h.x0 = x0;
+        h.x1 = x1;
+        h.x2 = x2;
+        h.hitflag = hitflag ? 1 : 0;
+        h.pad = 0;
+        list[slot] = h;
+        phase = PH_IDLE;
+        ray = -1;
     }
 
     __device__ __forceinline__ void consume(const TraceArgs& a, float d) {

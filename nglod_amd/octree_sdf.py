@@ -191,8 +191,7 @@ class OctreeSDF(nn.Module):
             return self._mixed[1], self._mixed[2]
         dev = self.features[0].fm.device
         grids_h = [torch.empty(f.fm.numel(), dtype=torch.float16, device=dev) for f in self.features]
-        din_pad = 32 if self.pos_invariant else 64
-        w0s = [torch.empty(self.hidden_dim * din_pad, dtype=torch.int16, device=dev) for _ in self.louts]
+        w0s = [torch.empty(self.hidden_dim * self.fdim, dtype=torch.int16, device=dev) for _ in self.louts]
         m = self.descriptor(_lib.FP32)
         for i, g in enumerate(grids_h):
             m.grid_h[i] = g.data_ptr()

@@ -142,20 +142,26 @@ class SphereTracer(BaseTracer):
     (sdf-net/lib/tracer/SphereTracer.py:41-132).
 
     form: 'fp32' (the reference's arithmetic) or 'mixed' (FP16 grids, BF16
-    MFMA decoder, last `refine_steps` steps re-evaluated in FP32)."""
+    MFMA decoder; when the march stops on a surface test its last
+    `refine_steps` evaluations are redone in FP32 and it goes on in FP32 to
+    the FP32 stop; normals are FP32).
 
-    def __init__(self, args=None, form: str = None, refine_steps: int = 4, **kw):
+    The tracer keeps one device workspace per device; concurrent calls of one
+    tracer object on different streams would share it and must not overlap."""
+
+    def __init__(self, args=None, form: str = None, refine_steps: int = 4, return_dist: float = 1e-2, **kw):
         super().__init__(args, **kw)
         self.form = form or getattr(args, "form", None) or "fp32"
         self.refine_steps = refine_steps
+        self.return_dist = return_dist
         self._ws = {}
         self.last_evals = None
 
-    def _workspace(self, device):
+    def _workspace(self, device, n, form):
         key = (device.type, device.index)
+        nbytes = _lib.load().nglod_trace_dense_workspace_size(n, form)
         ws = self._ws.get(key)
-        if ws is None:
-            nbytes = _lib.load().nglod_trace_dense_workspace_size()
+        if ws is None or ws.numel() < nbytes:
             ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
             self._ws[key] = ws
         return ws
@@ -178,7 +184,7 @@ class SphereTracer(BaseTracer):
         t = torch.empty(n, 1, device=dev, dtype=torch.float32)
         hit = torch.empty(n, device=dev, dtype=torch.bool)
         normal = torch.empty(n, 3, device=dev, dtype=torch.float32)
-        evals = torch.zeros(1, device=dev, dtype=torch.int64)
+        evals = torch.zeros(2, device=dev, dtype=torch.int64)
         form = _lib.MIXED if self.form == "mixed" else _lib.FP32
         p = _lib.NglodTraceParams()
         p.num_steps = int(self.num_steps)
@@ -187,8 +193,9 @@ class SphereTracer(BaseTracer):
         p.far_clip = float(self.camera_clamp[1])
         p.normals = 1
         p.refine_steps = int(self.refine_steps)
+        p.return_dist = float(self.return_dist)
         m = net.descriptor(form)
-        ws = self._workspace(dev)
+        ws = self._workspace(dev, n, form)
         lib = _lib.load()
         _lib.check(lib.nglod_trace_dense(m, _lib.ptr(o), _lib.ptr(d), n, lod, p, form, _lib.ptr(x), _lib.ptr(t),
                                          _lib.ptr(hit), _lib.ptr(normal), _lib.ptr(ws), ws.numel(), _lib.ptr(evals),

@@ -119,7 +119,7 @@ def _trace_both(net, o, d, lod, num_steps, dev, form="fp32"):
     rb = tracer(netg, gpu(o, dev), gpu(d, dev), ray_steps=steps)
     torch.cuda.synchronize()
     g = dict(x=rb.x.cpu().numpy(), t=rb.depth[:, 0].cpu().numpy(), hit=rb.hit.cpu().numpy(),
-             normal=rb.normal.cpu().numpy(), steps=steps.cpu().numpy(), evals=int(tracer.last_evals.item()))
+             normal=rb.normal.cpu().numpy(), steps=steps.cpu().numpy(), evals=int(tracer.last_evals.sum().item()))
     return res, g
 
 
