@@ -146,6 +146,34 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
  * (sdf-net/lib/models/SOL_NGLOD.py:33-46), kept on device. */
 int nglod_pack_mixed(const nglod_model_t* model, void* stream);
 
+/* ---- training targets and samples ------------------------------------- */
+
+/* Workspace bytes nglod_mesh2sdf_f64 needs for nf triangles (per-triangle
+ * FP64 table). */
+size_t nglod_mesh2sdf_workspace_size(int64_t nf);
+
+/* Signed distance from points [n][3] to a closed triangle mesh [nf][3][3],
+ * computed in FP64.  Replaces mesh2sdf.mesh2sdf_gpu
+ * (sdf-net/lib/extensions/mesh2sdf_cuda/mesh2sdf_kernel.cu:895-927, bound at
+ * :1009, called from sdf-net/lib/torchgp/compute_sdf.py:38): unsigned distance
+ * by the edge/face rule of :423-445, negative iff all 13 stab directions
+ * (:341-353) have a hit on both sides of the point (:479-529, :584-602).
+ * dist [n] fp32; dist64 [n] fp64 (may be NULL). */
+int nglod_mesh2sdf_f64(const float* points, int64_t n, const float* mesh, int64_t nf, float* dist,
+                       double* dist64, void* workspace, size_t workspace_bytes, void* stream);
+
+/* n points uniform in [-1,1]^3 (sdf-net/lib/torchgp/sample_uniform.py:25-31),
+ * from a counter-based SplitMix64 stream: sample i depends on (seed, i) only. */
+int nglod_sample_uniform(int64_t n, uint64_t seed, float* out, void* stream);
+
+/* n area-weighted surface samples of mesh [nf][3][3] with barycentrics
+ * u = sqrt(r1), v = r2 (sdf-net/lib/torchgp/sample_surface.py:27-52), plus
+ * sigma * N(0,1) per axis when sigma > 0 (sample_near_surface.py:27-45).
+ * cdf [nf] fp64: inclusive normalized cumulative face areas
+ * (area_weighted_distribution.py:26-45).  face [n] (may be NULL): chosen face. */
+int nglod_sample_surface(const float* mesh, const double* cdf, int64_t nf, int64_t n, uint64_t seed,
+                         double sigma, float* out, int32_t* face, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,10 @@ SIGNATURES = {
     ),
     "nglod_normals_fd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _p, _p]),
     "nglod_pack_mixed": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p]),
+    "nglod_mesh2sdf_workspace_size": (_sz, [_i64]),
+    "nglod_mesh2sdf_f64": (ctypes.c_int, [_p, _i64, _p, _i64, _p, _p, _p, _sz, _p]),
+    "nglod_sample_uniform": (ctypes.c_int, [_i64, ctypes.c_uint64, _p, _p]),
+    "nglod_sample_surface": (ctypes.c_int, [_p, _p, _i64, _i64, ctypes.c_uint64, ctypes.c_double, _p, _p, _p]),
 }
 
 _lib = None

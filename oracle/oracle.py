@@ -171,3 +171,51 @@ def normals(model, x, lod, prec="f32"):
     getattr(lib(), f"orc_normals_{prec}")(ctypes.byref(model.struct()), _p(x), ctypes.c_int64(x.shape[0]),
                                           ctypes.c_int(lod), _p(out))
     return out
+
+
+# ---- training targets ------------------------------------------------------
+def mesh2sdf(pts, tris):
+    """FP64 signed distance (restated mesh2sdf) of pts [N,3] to tris [F,3,3]."""
+    p = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+    out = np.empty(p.shape[0], dtype=np.float64)
+    lib().orc_mesh2sdf_f64(_p(p), ctypes.c_int64(p.shape[0]), _p(t), ctypes.c_int64(t.shape[0]), _p(out))
+    return out
+
+
+def point_tri_dist2(p, a, b, c):
+    f = lib().orc_point_tri_dist2_f64
+    f.restype = ctypes.c_double
+    arr = [np.ascontiguousarray(v, dtype=np.float64) for v in (p, a, b, c)]
+    return f(*[_p(v) for v in arr])
+
+
+def winding(p, tris):
+    f = lib().orc_winding_f64
+    f.restype = ctypes.c_double
+    pp = np.ascontiguousarray(p, dtype=np.float64)
+    t = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+    return f(_p(pp), _p(t), ctypes.c_int64(t.shape[0]))
+
+
+def splitmix64(x):
+    f = lib().orc_splitmix64
+    f.restype = ctypes.c_uint64
+    f.argtypes = [ctypes.c_uint64]
+    return f(x)
+
+
+def sample_uniform(seed, n):
+    out = np.empty((n, 3), dtype=np.float32)
+    lib().orc_sample_uniform(ctypes.c_uint64(seed), ctypes.c_int64(n), _p(out))
+    return out
+
+
+def sample_surface(tris, cdf, seed, n, sigma):
+    t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    c = np.ascontiguousarray(cdf, dtype=np.float64)
+    out = np.empty((n, 3), dtype=np.float32)
+    face = np.empty(n, dtype=np.int32)
+    lib().orc_sample_surface(_p(t), _p(c), ctypes.c_int64(t.shape[0]), ctypes.c_uint64(seed), ctypes.c_int64(n),
+                             ctypes.c_double(sigma), _p(out), _p(face))
+    return out, face
