@@ -146,6 +146,36 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
  * (sdf-net/lib/models/SOL_NGLOD.py:33-46), kept on device. */
 int nglod_pack_mixed(const nglod_model_t* model, void* stream);
 
+/* ---- training: backward of the fused query ------------------------------ */
+
+/* Parameter gradients, device pointers laid out like nglod_model_t:
+ * grid[i] corner-major [(res[i]+1)^3][feature_dim] (= fm.grad in
+ * channels_last_3d), w0[i] [hidden][din], b0[i] [hidden], w1[i] [hidden],
+ * b1[i] [1]. */
+typedef struct nglod_model_grads {
+    float* grid[NGLOD_MAX_LODS];
+    float* w0[NGLOD_MAX_LODS];
+    float* b0[NGLOD_MAX_LODS];
+    float* w1[NGLOD_MAX_LODS];
+    float* b1[NGLOD_MAX_LODS];
+} nglod_model_grads_t;
+
+/* Workspace bytes nglod_sdf_bwd needs for n queries up to `lod`. */
+size_t nglod_sdf_bwd_workspace_size(const nglod_model_t* model, int64_t n, int32_t lod);
+
+/* Backward of OctreeSDF.sdf for levels 0..lod (the graph autograd builds
+ * through F.grid_sample and nn.Linear in Trainer.step_geometry,
+ * sdf-net/lib/trainer.py:317-339).  grad_out [lod+1][n]: dL/dpred_i; only
+ * levels whose bit is set in level_mask are read.  Writes (does not
+ * accumulate) grads->grid[0..lod] and the decoders' gradients.  Interpolation
+ * weights and decoder activations are recomputed, not stored by the forward.
+ * Deterministic: fixed-order reductions, grid gradients by stable radix sort
+ * of (corner, query) keys and an in-order segmented sum; no float atomics.
+ * Position gradients are not computed.  n <= 2^28. */
+int nglod_sdf_bwd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
+                  const float* grad_out, int32_t level_mask, const nglod_model_grads_t* grads,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---- training targets and samples ------------------------------------- */
 
 /* Workspace bytes nglod_mesh2sdf_f64 needs for nf triangles (per-triangle

@@ -71,6 +71,12 @@ class NglodTraceParams(ctypes.Structure):
     ]
 
 
+class NglodModelGrads(ctypes.Structure):
+    """Mirror of nglod_model_grads_t."""
+
+    _fields_ = [(name, _p * MAX_LODS) for name in ("grid", "w0", "b0", "w1", "b1")]
+
+
 # name -> (restype, argtypes); every symbol include/nglod_hip.h declares.
 SIGNATURES = {
     "nglod_abi_version": (ctypes.c_int, []),
@@ -87,6 +93,9 @@ SIGNATURES = {
     ),
     "nglod_normals_fd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _p, _p]),
     "nglod_pack_mixed": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p]),
+    "nglod_sdf_bwd_workspace_size": (_sz, [ctypes.POINTER(NglodModel), _i64, _i32]),
+    "nglod_sdf_bwd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _p, _i32,
+                                      ctypes.POINTER(NglodModelGrads), _p, _sz, _p]),
     "nglod_mesh2sdf_workspace_size": (_sz, [_i64]),
     "nglod_mesh2sdf_f64": (ctypes.c_int, [_p, _i64, _p, _i64, _p, _p, _p, _sz, _p]),
     "nglod_sample_uniform": (ctypes.c_int, [_i64, ctypes.c_uint64, _p, _p]),

@@ -1,7 +1,25 @@
-"""torch.autograd.Function around the fused HIP query (OctreeSDF.sdf)."""
+"""torch.autograd.Function around the fused HIP query (OctreeSDF.sdf) with the
+hand-written backward (nglod_sdf_bwd): parameter gradients only, computed
+deterministically (no float atomics)."""
 import torch
 
 from . import _lib
+
+
+def _param_slots(net):
+    """(kind, index, name) of every parameter in net.parameters() order."""
+    slots = []
+    for name, _ in net.named_parameters():
+        parts = name.split(".")
+        if parts[0] == "features":
+            slots.append(("grid", int(parts[1]), None))
+        elif parts[0] == "louts":
+            key = {("0", "weight"): "w0", ("0", "bias"): "b0", ("2", "weight"): "w1", ("2", "bias"): "b1"}[
+                (parts[2], parts[3])]
+            slots.append((key, int(parts[1]), None))
+        else:
+            slots.append((None, -1, name))
+    return slots
 
 
 class _SdfFunction(torch.autograd.Function):
@@ -10,18 +28,59 @@ class _SdfFunction(torch.autograd.Function):
         ctx.net = net
         ctx.lod = lod
         ctx.all_levels = all_levels
-        ctx.save_for_backward(x)
+        xf = x.detach().reshape(-1, 3).float().contiguous()
+        ctx.save_for_backward(xf)
         if all_levels:
-            return torch.stack(net.sdf_all_hip(x, lod), 0)
-        return net.sdf_hip(x, lod)
+            return torch.stack(net.sdf_all_hip(xf, lod), 0)
+        return net.sdf_hip(xf, lod)
 
     @staticmethod
     def backward(ctx, grad_out):
-        raise NotImplementedError("backward of the fused query is not available in this build")
+        if ctx.needs_input_grad[1]:
+            raise NotImplementedError("gradients with respect to query positions are not computed")
+        (x,) = ctx.saved_tensors
+        net, L = ctx.net, ctx.lod
+        n = x.shape[0]
+        dev = x.device
+        if ctx.all_levels:
+            g = grad_out[..., 0].float().contiguous()  # [L+1, n]
+            mask = (1 << (L + 1)) - 1
+        else:
+            g = torch.zeros(L + 1, n, dtype=torch.float32, device=dev)
+            g[L] = grad_out[:, 0].float()
+            mask = 1 << L
+        used_dec = {0} if net.num_decoder == 1 else {i for i in range(L + 1) if (mask >> i) & 1}
+        G = _lib.NglodModelGrads()
+        out = {}
+        for i, f in enumerate(net.features):
+            if i <= L:
+                t = torch.empty_like(f.fm, memory_format=torch.channels_last_3d)
+                out[("grid", i)] = t
+                G.grid[i] = t.data_ptr()
+        for d, l in enumerate(net.louts):
+            for key, p in (("w0", l[0].weight), ("b0", l[0].bias), ("w1", l[2].weight), ("b1", l[2].bias)):
+                t = torch.empty_like(p, memory_format=torch.contiguous_format)
+                out[(key, d)] = t
+                getattr(G, key)[d] = t.data_ptr()
+        m = net.descriptor(_lib.FP32)
+        lib = _lib.load()
+        nbytes = lib.nglod_sdf_bwd_workspace_size(m, n, L)
+        ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev)
+        _lib.check(lib.nglod_sdf_bwd(m, _lib.ptr(x), n, L, _lib.ptr(g), mask, G, _lib.ptr(ws), ws.numel(),
+                                     _lib.stream_of(x)), "nglod_sdf_bwd")
+        grads = []
+        for kind, idx, _ in _param_slots(net):
+            if kind == "grid":
+                grads.append(out.get(("grid", idx)))
+            elif kind is not None and idx in used_dec:
+                grads.append(out[(kind, idx)])
+            else:
+                grads.append(None)
+        return (None, None, None, None, *grads)
 
 
 def sdf_with_grad(net, x, lod, all_levels):
-    params = [p for p in net.parameters()]
+    params = list(net.parameters())
     out = _SdfFunction.apply(net, x, lod, all_levels, *params)
     if all_levels:
         return [out[i] for i in range(out.shape[0])]

@@ -1,0 +1,412 @@
+// Hand-written backward of the fused query (OctreeSDF.sdf, every level up
+// to lod) for training, deterministic by construction (no float atomics):
+//
+//   1. per-query kernel: recomputes the trilinear weights and running
+//      features, and for each level with an incoming gradient the decoder
+//      pre-activations Z = W0 e + b0; writes E (decoder input), Z and
+//      ds = W0^T dz (feature part), dz = g * W1 * [Z > 0].
+//   2. decoder gradients: each block reduces a fixed chunk of queries in
+//      query order into partial dW0, db0, dW1, db1; a second kernel sums the
+//      partials in chunk order (and over levels, in level order, for a
+//      joint decoder).
+//   3. grid gradients, per level j: every (query, corner) contribution
+//      w_c * dg_j with dg_j = sum_{i >= j} ds_i gets the corner index as a
+//      key; a stable radix sort groups equal corners with queries in
+//      ascending order; a segmented sum writes each corner's gradient once.
+//
+// Replaces autograd through F.grid_sample and nn.Linear in
+// Trainer.step_geometry (sdf-net/lib/trainer.py:317-339).
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "api_common.hpp"
+
+namespace nglod {
+
+constexpr int kChunk = 2048;  // queries per partial of the decoder-gradient reduction
+
+struct BwdArgs {
+    ModelArgs m;
+    int lod;
+    int din;       // decoder input width (32 or 35)
+    int off;       // 0 or 3: offset of the features in the decoder input
+    int mask;      // bit i: level i has an incoming gradient
+    int64_t n;
+    const float* x;
+    const float* g;  // [lod+1][n]
+    float* E;        // [lod+1][n][din]
+    float* Z;        // [lod+1][n][128]
+    float* ds;       // [lod+1][n][32]
+};
+
+__global__ __launch_bounds__(256) void bwd_query_kernel(BwdArgs a) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.n) return;
+    const float px = a.x[3 * q], py = a.x[3 * q + 1], pz = a.x[3 * q + 2];
+    float f[kC];
+#pragma unroll
+    for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+    for (int l = 0; l <= a.lod; ++l) {
+        const int r = a.m.res[l];
+        const int S = r + 1;
+        const AxisW ax = axis_weights(px, r);
+        const AxisW ay = axis_weights(py, r);
+        const AxisW az = axis_weights(pz, r);
+        const float* base = a.m.grid[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
+        const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
+        const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
+        const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
+        corner_fma(f, base, wxy00 * az.w0);
+        corner_fma(f, base + sx, wxy10 * az.w0);
+        corner_fma(f, base + sy, wxy01 * az.w0);
+        corner_fma(f, base + sy + sx, wxy11 * az.w0);
+        corner_fma(f, base + sz, wxy00 * az.w1);
+        corner_fma(f, base + sz + sx, wxy10 * az.w1);
+        corner_fma(f, base + sz + sy, wxy01 * az.w1);
+        corner_fma(f, base + sz + sy + sx, wxy11 * az.w1);
+        float* E = a.E + ((size_t)l * a.n + q) * a.din;
+        if (a.off) {
+            E[0] = px;
+            E[1] = py;
+            E[2] = pz;
+        }
+#pragma unroll
+        for (int c = 0; c < kC; ++c) E[a.off + c] = f[c];
+        float* dsq = a.ds + ((size_t)l * a.n + q) * kC;
+        if (!((a.mask >> l) & 1)) {
+#pragma unroll
+            for (int c = 0; c < kC; ++c) dsq[c] = 0.0f;
+            continue;
+        }
+        const int dec = a.m.num_dec == 1 ? 0 : l;
+        const float* W0 = a.m.w0[dec];
+        const float* B0 = a.m.b0[dec];
+        const float* W1 = a.m.w1[dec];
+        const float gq = a.g[(size_t)l * a.n + q];
+        float de[kC];
+#pragma unroll
+        for (int c = 0; c < kC; ++c) de[c] = 0.0f;
+        float* Zq = a.Z + ((size_t)l * a.n + q) * kH;
+        for (int j = 0; j < kH; ++j) {
+            float acc = 0.0f;
+            if (a.off) {
+                acc = __builtin_fmaf(ldc(W0, j * a.din + 0), px, acc);
+                acc = __builtin_fmaf(ldc(W0, j * a.din + 1), py, acc);
+                acc = __builtin_fmaf(ldc(W0, j * a.din + 2), pz, acc);
+            }
+#pragma unroll
+            for (int k = 0; k < kC; ++k) acc = __builtin_fmaf(ldc(W0, j * a.din + a.off + k), f[k], acc);
+            const float z = acc + ldc(B0, j);
+            Zq[j] = z;
+            const float dz = z > 0.0f ? gq * ldc(W1, j) : 0.0f;
+#pragma unroll
+            for (int k = 0; k < kC; ++k) de[k] = __builtin_fmaf(dz, ldc(W0, j * a.din + a.off + k), de[k]);
+        }
+#pragma unroll
+        for (int c = 0; c < kC; ++c) dsq[c] = de[c];
+    }
+}
+
+// Partial decoder gradients of one level over one chunk of queries, in
+// query order.  Output layout per (level, chunk): dW0 [128][din], db0 [128],
+// dW1 [128], db1 [1].  Thread t owns hidden row j = t % 128 and half of the
+// input columns (t / 128).
+__global__ __launch_bounds__(256) void bwd_decoder_partial_kernel(BwdArgs a, int level, float* __restrict__ part,
+                                                                  int64_t nchunks) {
+    const int64_t chunk = blockIdx.x;
+    const int t = threadIdx.x;
+    const int j = t & (kH - 1);
+    const int half = t >> 7;
+    const int k0 = half ? (a.din + 1) / 2 : 0;
+    const int k1 = half ? a.din : (a.din + 1) / 2;
+    const int dec = a.m.num_dec == 1 ? 0 : level;
+    const float w1 = a.m.w1[dec][j];
+    float acc_w0[18];
+#pragma unroll
+    for (int k = 0; k < 18; ++k) acc_w0[k] = 0.0f;
+    float acc_b0 = 0.0f, acc_w1 = 0.0f, acc_b1 = 0.0f;
+    const int64_t q0 = chunk * kChunk;
+    const int64_t q1 = q0 + kChunk < a.n ? q0 + kChunk : a.n;
+    const float* Zl = a.Z + (size_t)level * a.n * kH;
+    const float* El = a.E + (size_t)level * a.n * a.din;
+    const float* gl = a.g + (size_t)level * a.n;
+    for (int64_t q = q0; q < q1; ++q) {
+        const float gq = gl[q];
+        const float z = Zl[q * kH + j];
+        const float h = z > 0.0f ? z : 0.0f;
+        const float dz = z > 0.0f ? gq * w1 : 0.0f;
+        const float* Eq = El + q * a.din;
+#pragma unroll
+        for (int k = 0; k < 18; ++k)
+            if (k0 + k < k1) acc_w0[k] = __builtin_fmaf(dz, Eq[k0 + k], acc_w0[k]);
+        acc_b0 += dz;
+        acc_w1 = __builtin_fmaf(gq, h, acc_w1);
+        acc_b1 += gq;
+    }
+    const int outs = kH * a.din + 2 * kH + 1;
+    float* P = part + ((size_t)level * nchunks + chunk) * outs;
+#pragma unroll
+    for (int k = 0; k < 18; ++k)
+        if (k0 + k < k1) P[j * a.din + k0 + k] = acc_w0[k];
+    if (half == 0) {
+        P[kH * a.din + j] = acc_b0;
+        P[kH * a.din + kH + j] = acc_w1;
+        if (j == 0) P[kH * a.din + 2 * kH] = acc_b1;
+    }
+}
+
+// Sums the partials of decoder `dec` over its levels (ascending) and chunks
+// (ascending) and writes dW0, db0, dW1, db1.
+__global__ void bwd_decoder_final_kernel(BwdArgs a, int dec, const float* __restrict__ part, int64_t nchunks,
+                                         float* __restrict__ dw0, float* __restrict__ db0, float* __restrict__ dw1,
+                                         float* __restrict__ db1) {
+    const int outs = kH * a.din + 2 * kH + 1;
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= outs) return;
+    float s = 0.0f;
+    for (int l = 0; l <= a.lod; ++l) {
+        const int d = a.m.num_dec == 1 ? 0 : l;
+        if (d != dec || !((a.mask >> l) & 1)) continue;
+        const float* P = part + (size_t)l * nchunks * outs + o;
+        for (int64_t c = 0; c < nchunks; ++c) s += P[c * outs];
+    }
+    if (o < kH * a.din) dw0[o] = s;
+    else if (o < kH * a.din + kH) db0[o - kH * a.din] = s;
+    else if (o < kH * a.din + 2 * kH) dw1[o - kH * a.din - kH] = s;
+    else db1[0] = s;
+}
+
+// Keys (corner index at `level`) and values (query * 8 + corner slot).
+__global__ void bwd_corner_keys_kernel(BwdArgs a, int level, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.n) return;
+    const int r = a.m.res[level];
+    const int S = r + 1;
+    const AxisW ax = axis_weights(a.x[3 * q], r);
+    const AxisW ay = axis_weights(a.x[3 * q + 1], r);
+    const AxisW az = axis_weights(a.x[3 * q + 2], r);
+    const uint32_t base = (uint32_t)((az.i0 * S + ay.i0) * S + ax.i0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const uint32_t off = (uint32_t)((c & 1) + ((c >> 1) & 1) * S + ((c >> 2) & 1) * S * S);
+        keys[q * 8 + c] = base + off;
+        vals[q * 8 + c] = (uint32_t)(q * 8 + c);
+    }
+}
+
+// One thread per sorted entry; the first entry of each run of equal corners
+// sums the run (queries ascending) and writes that corner's 32 gradients.
+__global__ void bwd_corner_sum_kernel(BwdArgs a, int level, const uint32_t* __restrict__ keys,
+                                      const uint32_t* __restrict__ vals, int64_t m, float* __restrict__ grad) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    const uint32_t key = keys[e];
+    if (e > 0 && keys[e - 1] == key) return;
+    const int r = a.m.res[level];
+    float acc[kC];
+#pragma unroll
+    for (int c = 0; c < kC; ++c) acc[c] = 0.0f;
+    for (int64_t i = e; i < m && keys[i] == key; ++i) {
+        const uint32_t v = vals[i];
+        const int64_t q = v >> 3;
+        const int c = v & 7;
+        const AxisW ax = axis_weights(a.x[3 * q], r);
+        const AxisW ay = axis_weights(a.x[3 * q + 1], r);
+        const AxisW az = axis_weights(a.x[3 * q + 2], r);
+        const float wx = (c & 1) ? ax.w1 : ax.w0;
+        const float wy = ((c >> 1) & 1) ? ay.w1 : ay.w0;
+        const float wz = ((c >> 2) & 1) ? az.w1 : az.w0;
+        const float w = (wx * wy) * wz;
+        float dg[kC];
+#pragma unroll
+        for (int k = 0; k < kC; ++k) dg[k] = 0.0f;
+        for (int l = a.lod; l >= level; --l) {
+            const float4* d4 = reinterpret_cast<const float4*>(a.ds + ((size_t)l * a.n + q) * kC);
+#pragma unroll
+            for (int k = 0; k < kC / 4; ++k) {
+                const float4 d = d4[k];
+                dg[4 * k] += d.x;
+                dg[4 * k + 1] += d.y;
+                dg[4 * k + 2] += d.z;
+                dg[4 * k + 3] += d.w;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kC; ++k) acc[k] = __builtin_fmaf(w, dg[k], acc[k]);
+    }
+    float4* g4 = reinterpret_cast<float4*>(grad + (size_t)key * kC);
+#pragma unroll
+    for (int k = 0; k < kC / 4; ++k) g4[k] = make_float4(acc[4 * k], acc[4 * k + 1], acc[4 * k + 2], acc[4 * k + 3]);
+}
+
+__global__ void fill_f32_kernel(float* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+struct BwdLayout {
+    size_t E, Z, ds, part, keys_in, keys_out, vals_in, vals_out, sort_tmp, total;
+    size_t sort_bytes;
+    int64_t nchunks;
+};
+
+static int key_bits(int r) {
+    const uint64_t corners = (uint64_t)(r + 1) * (r + 1) * (r + 1);
+    int b = 1;
+    while ((1ull << b) < corners) ++b;
+    return b;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+static int bwd_layout(int64_t n, int lod, int din, const int32_t* res, BwdLayout* L) {
+    const size_t levels = (size_t)lod + 1;
+    L->nchunks = (n + kChunk - 1) / kChunk;
+    const size_t outs = (size_t)kH * din + 2 * kH + 1;
+    size_t off = 0;
+    L->E = off;
+    off = align256(off + levels * n * din * sizeof(float));
+    L->Z = off;
+    off = align256(off + levels * n * kH * sizeof(float));
+    L->ds = off;
+    off = align256(off + levels * n * kC * sizeof(float));
+    L->part = off;
+    off = align256(off + levels * L->nchunks * outs * sizeof(float));
+    const size_t m = (size_t)n * 8;
+    L->keys_in = off;
+    off = align256(off + m * 4);
+    L->keys_out = off;
+    off = align256(off + m * 4);
+    L->vals_in = off;
+    off = align256(off + m * 4);
+    L->vals_out = off;
+    off = align256(off + m * 4);
+    size_t tmp = 0;
+    for (int j = 0; j <= lod; ++j) {
+        size_t t = 0;
+        if (rocprim::radix_sort_pairs((void*)nullptr, t, (uint32_t*)nullptr, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                      (uint32_t*)nullptr, m, 0, key_bits(res[j]), (hipStream_t)0) != hipSuccess)
+            return fail(NGLOD_ERR_HIP, "radix sort size query failed");
+        tmp = t > tmp ? t : tmp;
+    }
+    L->sort_tmp = off;
+    L->sort_bytes = tmp;
+    off = align256(off + tmp);
+    L->total = off;
+    return NGLOD_OK;
+}
+
+}  // namespace nglod
+
+using namespace nglod;
+
+extern "C" {
+
+size_t nglod_sdf_bwd_workspace_size(const nglod_model_t* model, int64_t n, int32_t lod) {
+    if (!model || n < 0 || lod < 0) return 0;
+    if (lod >= model->num_lods || lod >= NGLOD_MAX_LODS) return 0;
+    BwdLayout L;
+    const int din = model->pos_invariant ? kC : kC + 3;
+    if (bwd_layout(n, lod, din, model->res, &L)) return 0;
+    return L.total;
+}
+
+int nglod_sdf_bwd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod, const float* grad_out,
+                  int32_t level_mask, const nglod_model_grads_t* grads, void* workspace, size_t workspace_bytes,
+                  void* stream) {
+    ModelArgs m;
+    int rc = make_model_args(model, lod, NGLOD_FP32, &m);
+    if (rc) return rc;
+    if (n < 0) return fail(NGLOD_ERR_ARG, "n < 0");
+    if (n > ((int64_t)1 << 28)) return fail(NGLOD_ERR_ARG, "n too large for 32-bit sort values");
+    if (!grads) return fail(NGLOD_ERR_ARG, "grads is NULL");
+    const int din = m.pos_inv ? kC : kC + 3;
+    hipStream_t s = (hipStream_t)stream;
+    level_mask &= (1 << (lod + 1)) - 1;
+    // zero every gradient this call owns
+    for (int l = 0; l <= lod; ++l) {
+        if (!grads->grid[l] || !aligned16(grads->grid[l])) return fail(NGLOD_ERR_ARG, "grid grad NULL or misaligned");
+        const int64_t cnt = (int64_t)(m.res[l] + 1) * (m.res[l] + 1) * (m.res[l] + 1) * kC;
+        hipLaunchKernelGGL(fill_f32_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, grads->grid[l], cnt);
+    }
+    for (int d = 0; d < m.num_dec; ++d) {
+        if (!grads->w0[d] || !grads->b0[d] || !grads->w1[d] || !grads->b1[d])
+            return fail(NGLOD_ERR_ARG, "decoder grad NULL");
+        hipLaunchKernelGGL(fill_f32_kernel, dim3((kH * din + 255) / 256), dim3(256), 0, s, grads->w0[d], (int64_t)kH * din);
+        hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(256), 0, s, grads->b0[d], (int64_t)kH);
+        hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(256), 0, s, grads->w1[d], (int64_t)kH);
+        hipLaunchKernelGGL(fill_f32_kernel, dim3(1), dim3(64), 0, s, grads->b1[d], (int64_t)1);
+    }
+    rc = check_launch("fill_f32_kernel");
+    if (rc) return rc;
+    if (n == 0 || level_mask == 0) return NGLOD_OK;
+    if (!x || !grad_out) return fail(NGLOD_ERR_ARG, "NULL pointer");
+    BwdLayout L;
+    rc = bwd_layout(n, lod, din, model->res, &L);
+    if (rc) return rc;
+    if (!workspace || workspace_bytes < L.total || !aligned16(workspace))
+        return fail(NGLOD_ERR_WORKSPACE, "workspace too small or misaligned");
+    char* ws = (char*)workspace;
+    BwdArgs a;
+    a.m = m;
+    a.lod = lod;
+    a.din = din;
+    a.off = m.pos_inv ? 0 : 3;
+    a.mask = level_mask;
+    a.n = n;
+    a.x = x;
+    a.g = grad_out;
+    a.E = (float*)(ws + L.E);
+    a.Z = (float*)(ws + L.Z);
+    a.ds = (float*)(ws + L.ds);
+    float* part = (float*)(ws + L.part);
+    const unsigned qblocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(bwd_query_kernel, dim3(qblocks), dim3(256), 0, s, a);
+    rc = check_launch("bwd_query_kernel");
+    if (rc) return rc;
+    // decoder gradients
+    for (int l = 0; l <= lod; ++l) {
+        if (!((level_mask >> l) & 1)) continue;
+        hipLaunchKernelGGL(bwd_decoder_partial_kernel, dim3((unsigned)L.nchunks), dim3(256), 0, s, a, l, part, L.nchunks);
+    }
+    rc = check_launch("bwd_decoder_partial_kernel");
+    if (rc) return rc;
+    const int outs = kH * din + 2 * kH + 1;
+    for (int d = 0; d < m.num_dec; ++d) {
+        bool used = false;
+        for (int l = 0; l <= lod; ++l) used |= ((m.num_dec == 1 ? 0 : l) == d) && ((level_mask >> l) & 1);
+        if (!used) continue;
+        hipLaunchKernelGGL(bwd_decoder_final_kernel, dim3((outs + 255) / 256), dim3(256), 0, s, a, d, part, L.nchunks,
+                           grads->w0[d], grads->b0[d], grads->w1[d], grads->b1[d]);
+    }
+    rc = check_launch("bwd_decoder_final_kernel");
+    if (rc) return rc;
+    // grid gradients: levels that receive gradient from some level i >= j
+    uint32_t* kin = (uint32_t*)(ws + L.keys_in);
+    uint32_t* kout = (uint32_t*)(ws + L.keys_out);
+    uint32_t* vin = (uint32_t*)(ws + L.vals_in);
+    uint32_t* vout = (uint32_t*)(ws + L.vals_out);
+    const int64_t mm = n * 8;
+    for (int j = 0; j <= lod; ++j) {
+        if ((level_mask >> j) == 0) continue;  // no level >= j carries gradient
+        hipLaunchKernelGGL(bwd_corner_keys_kernel, dim3(qblocks), dim3(256), 0, s, a, j, kin, vin);
+        rc = check_launch("bwd_corner_keys_kernel");
+        if (rc) return rc;
+        size_t tmp = 0;
+        if (rocprim::radix_sort_pairs((void*)nullptr, tmp, kin, kout, vin, vout, (size_t)mm, 0, key_bits(m.res[j]), s) !=
+                hipSuccess || tmp > L.sort_bytes)
+            return fail(NGLOD_ERR_HIP, "radix sort temp size");
+        tmp = L.sort_bytes;
+        if (rocprim::radix_sort_pairs((void*)(ws + L.sort_tmp), tmp, kin, kout, vin, vout, (size_t)mm, 0,
+                                      key_bits(m.res[j]), s) != hipSuccess)
+            return fail(NGLOD_ERR_HIP, "radix sort failed");
+        hipLaunchKernelGGL(bwd_corner_sum_kernel, dim3((unsigned)((mm + 255) / 256)), dim3(256), 0, s, a, j, kout, vout,
+                           mm, grads->grid[j]);
+        rc = check_launch("bwd_corner_sum_kernel");
+        if (rc) return rc;
+    }
+    return NGLOD_OK;
+}
+
+}  // extern "C"
