@@ -78,6 +78,7 @@ def main():
     ap.add_argument("--lod", type=int, default=4)
     ap.add_argument("--num-steps", type=int, default=128)
     ap.add_argument("--cpu-sample", type=int, default=65536, help="rays in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--fit-iters", type=int, default=200, help="Adam steps of the seeded model fit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -86,10 +87,14 @@ def main():
 
     import torch
 
-    from nglod_amd import SphereTracer, _lib, workloads
+    from nglod_amd import SphereTracer, trainer, workloads
 
     dev = torch.device("cuda:0")
-    net = workloads.bench_model(device=dev)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter()
+    net, md = trainer.fitted_icosphere_model(seed=0, device=dev, iterations=args.fit_iters)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t_fit
     net.lod = args.lod
     o, d = workloads.camera_rays(args.width, args.height, eye=(0.0, 0.0, 3.0), device=dev)
     n = o.shape[0]
@@ -148,11 +153,14 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"sphere trace {args.width}x{args.height} pinhole frame, LOD {args.lod}, "
-                        f"max {args.num_steps} steps, min_dis 3e-4, FD normals; {workloads.BENCH_MODEL_NAME}",
+                        f"max {args.num_steps} steps, min_dis 3e-4, FD normals; 5-LOD OctreeSDF (r 4..64, C 32, "
+                        f"32->128->1) fitted {args.fit_iters} Adam steps (lr 1e-3, 2^17 samples) to the seed-0 "
+                        f"displaced level-4 icosphere",
             "rays": n,
             "lod": args.lod,
             "form": primary,
             "forms": res,
+            "model_fit_s": t_fit,
         },
         "roofline": {
             "bound": "hbm",

@@ -12,7 +12,10 @@
 //   3. grid gradients, per level j: every (query, corner) contribution
 //      w_c * dg_j with dg_j = sum_{i >= j} ds_i gets the corner index as a
 //      key; a stable radix sort groups equal corners with queries in
-//      ascending order; a segmented sum writes each corner's gradient once.
+//      ascending order; run-length encoding finds each corner's run; runs
+//      are cut into fixed pieces summed in order by one thread each, and the
+//      piece sums of a run are added in order (coarse levels have runs of
+//      thousands of contributions, so a run is not left to one thread).
 //
 // Replaces autograd through F.grid_sample and nn.Linear in
 // Trainer.step_geometry (sdf-net/lib/trainer.py:317-339).
@@ -24,7 +27,8 @@
 
 namespace nglod {
 
-constexpr int kChunk = 2048;  // queries per partial of the decoder-gradient reduction
+constexpr int kChunk = 256;   // queries per partial of the decoder-gradient reduction
+constexpr int kRunPiece = 64;  // sorted contributions per partial of a corner's gradient
 
 struct BwdArgs {
     ModelArgs m;
@@ -195,49 +199,86 @@ __global__ void bwd_corner_keys_kernel(BwdArgs a, int level, uint32_t* __restric
     }
 }
 
-// One thread per sorted entry; the first entry of each run of equal corners
-// sums the run (queries ascending) and writes that corner's 32 gradients.
-__global__ void bwd_corner_sum_kernel(BwdArgs a, int level, const uint32_t* __restrict__ keys,
-                                      const uint32_t* __restrict__ vals, int64_t m, float* __restrict__ grad) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= m) return;
-    const uint32_t key = keys[e];
-    if (e > 0 && keys[e - 1] == key) return;
+// Contribution of sorted entry e: w_c(x_q) * dg_j(q), dg_j = sum_{l >= j} ds_l.
+__device__ __forceinline__ void corner_contrib(const BwdArgs& a, int level, uint32_t v, float (&acc)[kC]) {
     const int r = a.m.res[level];
+    const int64_t q = v >> 3;
+    const int c = v & 7;
+    const AxisW ax = axis_weights(a.x[3 * q], r);
+    const AxisW ay = axis_weights(a.x[3 * q + 1], r);
+    const AxisW az = axis_weights(a.x[3 * q + 2], r);
+    const float wx = (c & 1) ? ax.w1 : ax.w0;
+    const float wy = ((c >> 1) & 1) ? ay.w1 : ay.w0;
+    const float wz = ((c >> 2) & 1) ? az.w1 : az.w0;
+    const float w = (wx * wy) * wz;
+    float dg[kC];
+#pragma unroll
+    for (int k = 0; k < kC; ++k) dg[k] = 0.0f;
+    for (int l = a.lod; l >= level; --l) {
+        const float4* d4 = reinterpret_cast<const float4*>(a.ds + ((size_t)l * a.n + q) * kC);
+#pragma unroll
+        for (int k = 0; k < kC / 4; ++k) {
+            const float4 d = d4[k];
+            dg[4 * k] += d.x;
+            dg[4 * k + 1] += d.y;
+            dg[4 * k + 2] += d.z;
+            dg[4 * k + 3] += d.w;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kC; ++k) acc[k] = __builtin_fmaf(w, dg[k], acc[k]);
+}
+
+// Pieces: run r (a run of equal corner keys, start rstart[r], length cnt[r])
+// is cut into ceil(cnt / kRunPiece) pieces; piece_start[r] is the exclusive
+// scan of those piece counts.  One thread per piece sums its entries in order.
+__global__ void run_pieces_kernel(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ nruns,
+                                  uint32_t* __restrict__ pieces) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= *nruns) return;
+    pieces[r] = (cnt[r] + kRunPiece - 1) / kRunPiece;
+}
+
+__global__ void bwd_piece_sum_kernel(BwdArgs a, int level, const uint32_t* __restrict__ vals,
+                                     const uint32_t* __restrict__ rstart, const uint32_t* __restrict__ cnt,
+                                     const uint32_t* __restrict__ piece_start, const uint32_t* __restrict__ nruns,
+                                     uint32_t max_pieces, float* __restrict__ partial) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t R = *nruns;
+    if (R == 0 || p >= max_pieces) return;
+    const uint32_t total = piece_start[R - 1] + (cnt[R - 1] + kRunPiece - 1) / kRunPiece;
+    if (p >= total) return;
+    // run of this piece: last r with piece_start[r] <= p
+    uint32_t lo = 0, hi = R - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (piece_start[mid] <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint32_t k = p - piece_start[lo];
+    const uint32_t b = rstart[lo] + k * kRunPiece;
+    const uint32_t e_end = min(rstart[lo] + cnt[lo], b + kRunPiece);
     float acc[kC];
 #pragma unroll
     for (int c = 0; c < kC; ++c) acc[c] = 0.0f;
-    for (int64_t i = e; i < m && keys[i] == key; ++i) {
-        const uint32_t v = vals[i];
-        const int64_t q = v >> 3;
-        const int c = v & 7;
-        const AxisW ax = axis_weights(a.x[3 * q], r);
-        const AxisW ay = axis_weights(a.x[3 * q + 1], r);
-        const AxisW az = axis_weights(a.x[3 * q + 2], r);
-        const float wx = (c & 1) ? ax.w1 : ax.w0;
-        const float wy = ((c >> 1) & 1) ? ay.w1 : ay.w0;
-        const float wz = ((c >> 2) & 1) ? az.w1 : az.w0;
-        const float w = (wx * wy) * wz;
-        float dg[kC];
+    for (uint32_t e = b; e < e_end; ++e) corner_contrib(a, level, vals[e], acc);
+    float4* o4 = reinterpret_cast<float4*>(partial + (size_t)p * kC);
 #pragma unroll
-        for (int k = 0; k < kC; ++k) dg[k] = 0.0f;
-        for (int l = a.lod; l >= level; --l) {
-            const float4* d4 = reinterpret_cast<const float4*>(a.ds + ((size_t)l * a.n + q) * kC);
-#pragma unroll
-            for (int k = 0; k < kC / 4; ++k) {
-                const float4 d = d4[k];
-                dg[4 * k] += d.x;
-                dg[4 * k + 1] += d.y;
-                dg[4 * k + 2] += d.z;
-                dg[4 * k + 3] += d.w;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kC; ++k) acc[k] = __builtin_fmaf(w, dg[k], acc[k]);
-    }
-    float4* g4 = reinterpret_cast<float4*>(grad + (size_t)key * kC);
-#pragma unroll
-    for (int k = 0; k < kC / 4; ++k) g4[k] = make_float4(acc[4 * k], acc[4 * k + 1], acc[4 * k + 2], acc[4 * k + 3]);
+    for (int c = 0; c < kC / 4; ++c) o4[c] = make_float4(acc[4 * c], acc[4 * c + 1], acc[4 * c + 2], acc[4 * c + 3]);
+}
+
+// One thread per (run, channel): sums the run's piece partials in order.
+__global__ void bwd_run_final_kernel(const uint32_t* __restrict__ ukeys, const uint32_t* __restrict__ cnt,
+                                     const uint32_t* __restrict__ piece_start, const uint32_t* __restrict__ nruns,
+                                     uint32_t max_runs, const float* __restrict__ partial, float* __restrict__ grad) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r = (uint32_t)(t / kC);
+    const int c = (int)(t % kC);
+    if (r >= max_runs || r >= *nruns) return;
+    const uint32_t np = (cnt[r] + kRunPiece - 1) / kRunPiece;
+    float s = 0.0f;
+    for (uint32_t i = 0; i < np; ++i) s += partial[(size_t)(piece_start[r] + i) * kC + c];
+    grad[(size_t)ukeys[r] * kC + c] = s;
 }
 
 __global__ void fill_f32_kernel(float* p, int64_t n) {
@@ -247,8 +288,10 @@ __global__ void fill_f32_kernel(float* p, int64_t n) {
 
 struct BwdLayout {
     size_t E, Z, ds, part, keys_in, keys_out, vals_in, vals_out, sort_tmp, total;
-    size_t sort_bytes;
+    size_t ukeys, counts, nruns, rstart, pstart, pcount, piece_part, rle_tmp;
+    size_t sort_bytes, rle_bytes;
     int64_t nchunks;
+    uint32_t max_runs, max_pieces;
 };
 
 static int key_bits(int r) {
@@ -293,6 +336,38 @@ static int bwd_layout(int64_t n, int lod, int din, const int32_t* res, BwdLayout
     L->sort_tmp = off;
     L->sort_bytes = tmp;
     off = align256(off + tmp);
+    // run-length encoding of the sorted keys and the run pieces
+    uint64_t max_corners = 0;
+    for (int j = 0; j <= lod; ++j) {
+        const uint64_t c = (uint64_t)(res[j] + 1) * (res[j] + 1) * (res[j] + 1);
+        max_corners = c > max_corners ? c : max_corners;
+    }
+    L->max_runs = (uint32_t)(max_corners < m ? max_corners : m);
+    L->max_pieces = (uint32_t)(m / kRunPiece + L->max_runs);
+    L->ukeys = off;
+    off = align256(off + (size_t)L->max_runs * 4);
+    L->counts = off;
+    off = align256(off + (size_t)L->max_runs * 4);
+    L->rstart = off;
+    off = align256(off + (size_t)L->max_runs * 4);
+    L->pcount = off;
+    off = align256(off + (size_t)L->max_runs * 4);
+    L->pstart = off;
+    off = align256(off + (size_t)L->max_runs * 4);
+    L->nruns = off;
+    off = align256(off + 16);
+    L->piece_part = off;
+    off = align256(off + (size_t)L->max_pieces * kC * sizeof(float));
+    size_t t1 = 0, t2 = 0, t3 = 0;
+    if (rocprim::run_length_encode((void*)nullptr, t1, (uint32_t*)nullptr, m, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                   (uint32_t*)nullptr, (hipStream_t)0) != hipSuccess ||
+        rocprim::exclusive_scan((void*)nullptr, t2, (uint32_t*)nullptr, (uint32_t*)nullptr, 0u,
+                                (size_t)L->max_runs, rocprim::plus<uint32_t>(), (hipStream_t)0) != hipSuccess)
+        return fail(NGLOD_ERR_HIP, "run-length / scan size query failed");
+    t3 = t1 > t2 ? t1 : t2;
+    L->rle_tmp = off;
+    L->rle_bytes = t3;
+    off = align256(off + t3);
     L->total = off;
     return NGLOD_OK;
 }
@@ -401,9 +476,36 @@ int nglod_sdf_bwd(const nglod_model_t* model, const float* x, int64_t n, int32_t
         if (rocprim::radix_sort_pairs((void*)(ws + L.sort_tmp), tmp, kin, kout, vin, vout, (size_t)mm, 0,
                                       key_bits(m.res[j]), s) != hipSuccess)
             return fail(NGLOD_ERR_HIP, "radix sort failed");
-        hipLaunchKernelGGL(bwd_corner_sum_kernel, dim3((unsigned)((mm + 255) / 256)), dim3(256), 0, s, a, j, kout, vout,
-                           mm, grads->grid[j]);
-        rc = check_launch("bwd_corner_sum_kernel");
+        // runs of equal corners -> pieces of kRunPiece entries -> in-order sums
+        uint32_t* ukeys = (uint32_t*)(ws + L.ukeys);
+        uint32_t* cnts = (uint32_t*)(ws + L.counts);
+        uint32_t* rstart = (uint32_t*)(ws + L.rstart);
+        uint32_t* pcount = (uint32_t*)(ws + L.pcount);
+        uint32_t* pstart = (uint32_t*)(ws + L.pstart);
+        uint32_t* nruns = (uint32_t*)(ws + L.nruns);
+        float* ppart = (float*)(ws + L.piece_part);
+        size_t t = L.rle_bytes;
+        if (rocprim::run_length_encode((void*)(ws + L.rle_tmp), t, kout, (size_t)mm, ukeys, cnts, nruns, s) != hipSuccess)
+            return fail(NGLOD_ERR_HIP, "run-length encode failed");
+        t = L.rle_bytes;
+        if (rocprim::exclusive_scan((void*)(ws + L.rle_tmp), t, cnts, rstart, 0u, (size_t)L.max_runs,
+                                    rocprim::plus<uint32_t>(), s) != hipSuccess)
+            return fail(NGLOD_ERR_HIP, "scan failed");
+        hipLaunchKernelGGL(run_pieces_kernel, dim3((L.max_runs + 255) / 256), dim3(256), 0, s, cnts, nruns, pcount);
+        rc = check_launch("run_pieces_kernel");
+        if (rc) return rc;
+        t = L.rle_bytes;
+        if (rocprim::exclusive_scan((void*)(ws + L.rle_tmp), t, pcount, pstart, 0u, (size_t)L.max_runs,
+                                    rocprim::plus<uint32_t>(), s) != hipSuccess)
+            return fail(NGLOD_ERR_HIP, "scan failed");
+        hipLaunchKernelGGL(bwd_piece_sum_kernel, dim3((L.max_pieces + 255) / 256), dim3(256), 0, s, a, j, vout, rstart,
+                           cnts, pstart, nruns, L.max_pieces, ppart);
+        rc = check_launch("bwd_piece_sum_kernel");
+        if (rc) return rc;
+        const uint64_t fin = (uint64_t)L.max_runs * kC;
+        hipLaunchKernelGGL(bwd_run_final_kernel, dim3((unsigned)((fin + 255) / 256)), dim3(256), 0, s, ukeys, cnts, pstart,
+                           nruns, L.max_runs, ppart, grads->grid[j]);
+        rc = check_launch("bwd_run_final_kernel");
         if (rc) return rc;
     }
     return NGLOD_OK;

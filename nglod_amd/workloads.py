@@ -105,11 +105,3 @@ def camera_rays(width, height, eye=(0.0, 0.0, 3.0), at=(0.0, 0.0, 0.0), fov=30.0
 def uniform_points(n, seed=1, device="cuda"):
     g = torch.Generator().manual_seed(seed)
     return (torch.rand(n, 3, generator=g) * 2.0 - 1.0).to(device)
-
-
-BENCH_MODEL_NAME = "5-LOD OctreeSDF (r 4..64, C 32, 32->128->1), sphere field r=0.6 in the finest grid + seeded noise"
-
-
-def bench_model(device="cuda"):
-    """The model the frame benchmark traces."""
-    return make_sphere_model(radius=0.6, num_lods=5, device=device)
