@@ -146,6 +146,51 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
  * (sdf-net/lib/models/SOL_NGLOD.py:33-46), kept on device. */
 int nglod_pack_mixed(const nglod_model_t* model, void* stream);
 
+/* ---- sparse octree (SPC) --------------------------------------------------
+ * Setup calls below are synchronous (they read sizes back) and follow one
+ * workspace protocol: if workspace_bytes is too small they return
+ * NGLOD_ERR_WORKSPACE and set *workspace_needed; call again with at least that
+ * much.  Output capacities too small return NGLOD_ERR_CAPACITY with the
+ * required count already written to the count out-parameter. */
+
+/* Level-`level` cells (1..10) that triangles [nf][3][3] overlap (closed-box
+ * separating-axis test in FP64): sorted unique 30-bit Morton codes (x in the
+ * high bit of each triple, SPC.h:65-80).  The benchmark octree's occupancy;
+ * the reference samples it from traced surface points instead
+ * (sdf-net/lib/renderutils.py:57-73). */
+int nglod_spc_occupancy(const float* tris, int64_t nf, int32_t level, uint32_t* codes, int64_t cap,
+                        int64_t* count, void* workspace, size_t workspace_bytes, size_t* workspace_needed,
+                        void* stream);
+
+/* Sorted unique leaf codes at `level` -> octree occupancy bytes, level 0
+ * first (SPC3D.points_to_nodes, sdf-net/lib/spc3d.py:224-255).  level_counts
+ * (host, level+1 entries): nodes per level. */
+int nglod_spc_from_codes(const uint32_t* codes, int64_t n, int32_t level, uint8_t* octree, int64_t cap,
+                         int64_t* osize, int32_t* level_counts, void* workspace, size_t workspace_bytes,
+                         size_t* workspace_needed, void* stream);
+
+/* Octree bytes -> points [P][4] int16 (x, y, z, 0) and the pyramid (host,
+ * [2][L+2]: row 0 points per level, row 1 exclusive sums) as SPC::SetGeometry
+ * (sol-renderer/include/spc/spc/SPC.cu:148-238).  prefix (device, osize+1):
+ * exclusive sums of the bytes' popcounts, needed by nglod_spc_raytrace. */
+int nglod_spc_points(const uint8_t* octree, int64_t osize, int16_t* points, int64_t pcap, int32_t* pyramid,
+                     int32_t pyramid_cap, int32_t* level_out, int64_t* psize_out, uint32_t* prefix,
+                     void* workspace, size_t workspace_bytes, size_t* workspace_needed, void* stream);
+
+/* Ray-octree traversal to level `target`: nuggets [M][2] int32 (ray, point
+ * index within level `target`), rays in ascending order, each ray's cells
+ * front-to-back as the reference orders them.  Replaces spc_raytrace
+ * (sol-renderer/include/spc/spc/spc_raytrace_cuda.cpp:141-199).  ray_offsets
+ * (device, n+1): CSR offsets of each ray's nuggets.  Bit-identical list. */
+int nglod_spc_raytrace(const uint8_t* octree, const uint32_t* prefix, const int16_t* points,
+                       const int32_t* pyramid, int32_t level, const float* ray_o, const float* ray_d, int64_t n,
+                       int32_t target, int32_t* nuggets, int64_t cap, int64_t* count, uint32_t* ray_offsets,
+                       void* workspace, size_t workspace_bytes, size_t* workspace_needed, void* stream);
+
+/* info[i] = 1 where nugget i starts a new ray (d_MarkUniqueRays,
+ * sol-renderer/sdfRenderer.cu:108-120). */
+int nglod_mark_unique_rays(const int32_t* nuggets, int64_t m, int32_t* info, void* stream);
+
 /* ---- training: backward of the fused query ------------------------------ */
 
 /* Parameter gradients, device pointers laid out like nglod_model_t:

@@ -96,6 +96,15 @@ SIGNATURES = {
     "nglod_sdf_bwd_workspace_size": (_sz, [ctypes.POINTER(NglodModel), _i64, _i32]),
     "nglod_sdf_bwd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _p, _i32,
                                       ctypes.POINTER(NglodModelGrads), _p, _sz, _p]),
+    "nglod_spc_occupancy": (ctypes.c_int, [_p, _i64, _i32, _p, _i64, ctypes.POINTER(_i64), _p, _sz,
+                                            ctypes.POINTER(_sz), _p]),
+    "nglod_spc_from_codes": (ctypes.c_int, [_p, _i64, _i32, _p, _i64, ctypes.POINTER(_i64), _p, _p, _sz,
+                                             ctypes.POINTER(_sz), _p]),
+    "nglod_spc_points": (ctypes.c_int, [_p, _i64, _p, _i64, _p, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_i64), _p,
+                                         _p, _sz, ctypes.POINTER(_sz), _p]),
+    "nglod_spc_raytrace": (ctypes.c_int, [_p, _p, _p, _p, _i32, _p, _p, _i64, _i32, _p, _i64, ctypes.POINTER(_i64),
+                                           _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "nglod_mark_unique_rays": (ctypes.c_int, [_p, _i64, _p, _p]),
     "nglod_mesh2sdf_workspace_size": (_sz, [_i64]),
     "nglod_mesh2sdf_f64": (ctypes.c_int, [_p, _i64, _p, _i64, _p, _p, _p, _sz, _p]),
     "nglod_sample_uniform": (ctypes.c_int, [_i64, ctypes.c_uint64, _p, _p]),

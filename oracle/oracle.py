@@ -219,3 +219,66 @@ def sample_surface(tris, cdf, seed, n, sigma):
     lib().orc_sample_surface(_p(t), _p(c), ctypes.c_int64(t.shape[0]), ctypes.c_uint64(seed), ctypes.c_int64(n),
                              ctypes.c_double(sigma), _p(out), _p(face))
     return out, face
+
+
+# ---- sparse octree ------------------------------------------------------------
+def morton(x, y, z):
+    f = lib().orc_morton
+    f.restype = ctypes.c_uint64
+    f.argtypes = [ctypes.c_uint32] * 3
+    return f(x, y, z)
+
+
+def occupancy(tris, level):
+    t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    f = lib().orc_occupancy
+    f.restype = ctypes.c_int64
+    cap = 1 << 22
+    out = np.empty(cap, dtype=np.uint64)
+    n = f(_p(t), ctypes.c_int64(t.shape[0]), ctypes.c_int(level), _p(out), ctypes.c_int64(cap))
+    assert n <= cap
+    return out[:n].copy()
+
+
+def octree_from_codes(codes, level):
+    c = np.ascontiguousarray(codes, dtype=np.uint64)
+    f = lib().orc_octree_from_codes
+    f.restype = ctypes.c_int64
+    cap = max(1, 2 * c.size + 16)
+    out = np.empty(cap, dtype=np.uint8)
+    counts = np.zeros(level + 1, dtype=np.int64)
+    n = f(_p(c), ctypes.c_int64(c.size), ctypes.c_int(level), _p(out), ctypes.c_int64(cap), _p(counts))
+    return out[:n].copy(), counts
+
+
+def octree_points(octree):
+    o = np.ascontiguousarray(octree, dtype=np.uint8)
+    psize = 1 + int(np.unpackbits(o).sum())
+    pts = np.empty((psize, 4), dtype=np.int16)
+    pyr = np.zeros(64, dtype=np.int32)
+    f = lib().orc_octree_points
+    L = f(_p(o), ctypes.c_int64(o.size), _p(pts), ctypes.c_int64(psize), _p(pyr), ctypes.c_int(64))
+    return pts, pyr[:2 * (L + 2)].reshape(2, L + 2).copy(), L
+
+
+def spc_raytrace(octree, points, pyramid, level, o, d, target):
+    oc = np.ascontiguousarray(octree, dtype=np.uint8)
+    pts = np.ascontiguousarray(points, dtype=np.int16)
+    pyr = np.ascontiguousarray(pyramid, dtype=np.int32).reshape(-1)
+    oo = np.ascontiguousarray(o, dtype=np.float32).reshape(-1, 3)
+    dd = np.ascontiguousarray(d, dtype=np.float32).reshape(-1, 3)
+    f = lib().orc_spc_raytrace
+    f.restype = ctypes.c_int64
+    cap = 1 << 24
+    out = np.empty((cap, 2), dtype=np.int32)
+    m = f(_p(oc), ctypes.c_int64(oc.size), _p(pts), _p(pyr), ctypes.c_int(level), _p(oo), _p(dd),
+          ctypes.c_int64(oo.shape[0]), ctypes.c_int(target), _p(out), ctypes.c_int64(cap))
+    assert m <= cap
+    return out[:m].copy()
+
+
+def mark_unique_rays(nuggets):
+    n = np.ascontiguousarray(nuggets, dtype=np.int32)
+    info = np.empty(n.shape[0], dtype=np.int32)
+    lib().orc_mark_unique_rays(_p(n), ctypes.c_int64(n.shape[0]), _p(info))
+    return info
