@@ -249,6 +249,96 @@ int nglod_sample_uniform(int64_t n, uint64_t seed, float* out, void* stream);
 int nglod_sample_surface(const float* mesh, const double* cdf, int64_t nf, int64_t n, uint64_t seed,
                          double sigma, float* out, int32_t* face, void* stream);
 
+/* n directions uniform on the unit sphere: z = 1 - 2u, phi = 2*pi*v
+ * (geoutils.sample_unif_sphere, sdf-net/lib/geoutils.py:31-40), same
+ * counter-based stream as nglod_sample_uniform. */
+int nglod_sample_directions(int64_t n, uint64_t seed, float* out, void* stream);
+
+/* ---- sparse ("SOL") model and its march ---------------------------------- */
+
+/* Sparse model: features live only on the corners of occupied octree cells.
+ * Levels 2..target of the octree carry LODs 0..lod (target = lod + 2,
+ * sdfRenderer.cu:220).  Replaces the state SDF::loadWeights and
+ * SDF::initTrinkets build (sol-renderer/SDF.cu:65-216).  All pointers are
+ * device pointers the caller owns. */
+typedef struct nglod_sol {
+    int32_t target;              /* octree level of the march cells (2..12) */
+    int32_t lod;                 /* = target - 2; levels lod..0 are summed */
+    int32_t din;                 /* decoder input: 35 (xyz first) or 32 */
+    int32_t reserved;
+    int64_t psum_target;         /* index of the first level-`target` point */
+    const int16_t* points;       /* [P][4] octree points, all levels */
+    const uint32_t* trinket_v;   /* [P][8] corner indices, 16-byte aligned */
+    const int32_t* parent;       /* [P] parent point index, -1 at level 2 */
+    const int32_t* cc;           /* [Nc][3] corner coordinates, per level
+                                    blocks sorted by (x, y, z) */
+    const float* cf;             /* [Nc][32] corner features */
+    const float* w0;             /* [128][din] decoder `lod` */
+    const float* b0;             /* [128] */
+    const float* w1;             /* [128] */
+    const float* b1;             /* [1] */
+} nglod_sol_t;
+
+/* Unique corners of the octree cells at levels 2..target, one block per
+ * level sorted by (x, y, z) as torch.unique(dim=0) orders them
+ * (geoutils.unique_corners, sdf-net/lib/geoutils.py:60-77).  pyramid: host
+ * [2][level+2] as nglod_spc_points returns it.  counts (host, target-1
+ * entries): corners per level; cc [cap][3].  Setup call (synchronous,
+ * workspace protocol above); cap too small returns NGLOD_ERR_CAPACITY with
+ * counts filled. */
+int nglod_sol_corners(const int16_t* points, const int32_t* pyramid, int32_t level, int32_t target, int32_t* cc,
+                      int64_t cap, int64_t* counts, void* workspace, size_t workspace_bytes,
+                      size_t* workspace_needed, void* stream);
+
+/* cf[c] = dense corner-major grid of LOD (level - 2) at cc[c]
+ * (feat[0, :, z, y, x] in unique_corners, geoutils.py:76), FP32.  Requires
+ * res[i] = 4 * 2^i. */
+int nglod_sol_features(const nglod_model_t* model, int32_t target, const int32_t* cc, const int64_t* counts,
+                       float* cf, void* stream);
+
+/* Per point of levels 2..target: the 8 corner indices (corner j at
+ * x + (j&4)>>2, y + (j&2)>>1, z + (j&1)) and the parent point
+ * (index_trinket_kernel, solr/sdf/index_trinket.cuh:31-98; binary searches
+ * instead of its O(n*m) scan).  trinket_v [P][8], parent [P]; entries of
+ * other levels are left untouched.  Synchronous; workspace >= 256 bytes.
+ * Returns NGLOD_ERR_ARG when a corner is not in cc or a parent is not in
+ * the octree (the reference would read out of bounds). */
+int nglod_sol_trinkets(const int16_t* points, const int32_t* pyramid, int32_t level, int32_t target,
+                       const int32_t* cc, const int64_t* counts, uint32_t* trinket_v, int32_t* parent,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* Sparse SDF at x [n][3] inside level-`target` cells pidx [n] (index within
+ * the level): sparse_grid_sample_kernel (solr/sdf/sparse_grid_sample.cuh:31-109)
+ * then the decoder (SDF.cu:420-421), fused.  out [n]. */
+int nglod_sol_sdf(const nglod_sol_t* sol, const float* x, const int32_t* pidx, int64_t n, float* out,
+                  void* stream);
+
+/* Workspace bytes of nglod_trace_sparse (a queue counter). */
+size_t nglod_trace_sparse_workspace_size(int64_t n);
+
+/* Sparse sphere march with normals (SDF::sphereTrace + SDF::getNormal,
+ * sol-renderer/SDF.cu:218-472, with ray_aabb_kernel and step_kernel): entry
+ * into the first nugget cell holding or ahead of the point, at most 50
+ * steps, hit when |d| < 3e-4 or |d + dprev|/2 < 1.5e-3, stop at t >= 5 or
+ * when no cell is left (t = 100); normals by central differences (1e-3) in
+ * the same cell, normalised; misses get normal (1, 1, 1).  ray_offsets [n+1]
+ * and nuggets [M][2] as nglod_spc_raytrace returns them at `target`.
+ * Outputs x [n][3], t [n], hit [n], normal [n][3], iters [n] (may be NULL).
+ * evals_out (device, may be NULL): SDF evaluations are added to it. */
+int nglod_trace_sparse(const nglod_sol_t* sol, const float* ray_o, const float* ray_d, int64_t n,
+                       const int32_t* ray_offsets, const int32_t* nuggets, float* x, float* t, uint8_t* hit,
+                       float* normal, int32_t* iters, void* workspace, size_t workspace_bytes,
+                       uint64_t* evals_out, void* stream);
+
+/* Sorted unique leaf codes at `level` of the cells holding points x [n][3]
+ * (where mask[i] != 0; mask may be NULL): the occupancy voxel_sparsify
+ * derives from traced surface points (sdf-net/lib/renderutils.py:57-73),
+ * cell = floor((x + 1) / 2 * 2^level) clamped to the grid.  Setup call
+ * (synchronous, workspace protocol above). */
+int nglod_spc_codes_from_points(const float* x, const uint8_t* mask, int64_t n, int32_t level, uint32_t* codes,
+                                int64_t cap, int64_t* count, void* workspace, size_t workspace_bytes,
+                                size_t* workspace_needed, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

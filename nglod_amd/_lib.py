@@ -77,6 +77,27 @@ class NglodModelGrads(ctypes.Structure):
     _fields_ = [(name, _p * MAX_LODS) for name in ("grid", "w0", "b0", "w1", "b1")]
 
 
+class NglodSol(ctypes.Structure):
+    """Mirror of nglod_sol_t."""
+
+    _fields_ = [
+        ("target", _i32),
+        ("lod", _i32),
+        ("din", _i32),
+        ("reserved", _i32),
+        ("psum_target", _i64),
+        ("points", _p),
+        ("trinket_v", _p),
+        ("parent", _p),
+        ("cc", _p),
+        ("cf", _p),
+        ("w0", _p),
+        ("b0", _p),
+        ("w1", _p),
+        ("b1", _p),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/nglod_hip.h declares.
 SIGNATURES = {
     "nglod_abi_version": (ctypes.c_int, []),
@@ -109,6 +130,16 @@ SIGNATURES = {
     "nglod_mesh2sdf_f64": (ctypes.c_int, [_p, _i64, _p, _i64, _p, _p, _p, _sz, _p]),
     "nglod_sample_uniform": (ctypes.c_int, [_i64, ctypes.c_uint64, _p, _p]),
     "nglod_sample_surface": (ctypes.c_int, [_p, _p, _i64, _i64, ctypes.c_uint64, ctypes.c_double, _p, _p, _p]),
+    "nglod_sample_directions": (ctypes.c_int, [_i64, ctypes.c_uint64, _p, _p]),
+    "nglod_sol_corners": (ctypes.c_int, [_p, _p, _i32, _i32, _p, _i64, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
+    "nglod_sol_features": (ctypes.c_int, [ctypes.POINTER(NglodModel), _i32, _p, _p, _p, _p]),
+    "nglod_sol_trinkets": (ctypes.c_int, [_p, _p, _i32, _i32, _p, _p, _p, _p, _p, _sz, _p]),
+    "nglod_sol_sdf": (ctypes.c_int, [ctypes.POINTER(NglodSol), _p, _p, _i64, _p, _p]),
+    "nglod_trace_sparse_workspace_size": (_sz, [_i64]),
+    "nglod_trace_sparse": (ctypes.c_int, [ctypes.POINTER(NglodSol), _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p,
+                                          _sz, _p, _p]),
+    "nglod_spc_codes_from_points": (ctypes.c_int, [_p, _p, _i64, _i32, _p, _i64, ctypes.POINTER(_i64), _p, _sz,
+                                                   ctypes.POINTER(_sz), _p]),
 }
 
 _lib = None

@@ -177,6 +177,19 @@ __global__ void sample_uniform_kernel(uint64_t seed, int64_t n, float* __restric
     for (int k = 0; k < 3; ++k) out[3 * i + k] = (float)(rnd01(seed, (uint64_t)i, (uint32_t)k) * 2.0 - 1.0);
 }
 
+// Directions uniform on S^2 (geoutils.sample_unif_sphere, sdf-net/lib/geoutils.py:31-40):
+// z = 1 - 2u, phi = 2*pi*v, (sqrt(1 - z^2) cos phi, sqrt(1 - z^2) sin phi, z).
+__global__ void sample_directions_kernel(uint64_t seed, int64_t n, float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double z = 1.0 - 2.0 * rnd01(seed, (uint64_t)i, 0);
+    const double r = sqrt(1.0 - z * z);
+    const double phi = 2.0 * M_PI * rnd01(seed, (uint64_t)i, 1);
+    out[3 * i] = (float)(r * cos(phi));
+    out[3 * i + 1] = (float)(r * sin(phi));
+    out[3 * i + 2] = (float)z;
+}
+
 __global__ void sample_surface_kernel(const float* __restrict__ mesh, const double* __restrict__ cdf, int64_t nf,
                                       uint64_t seed, int64_t n, double sigma, float* __restrict__ out,
                                       int32_t* __restrict__ face) {
@@ -241,6 +254,15 @@ int nglod_sample_uniform(int64_t n, uint64_t seed, float* out, void* stream) {
     hipLaunchKernelGGL(sample_uniform_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
                        seed, n, out);
     return check_launch("sample_uniform_kernel");
+}
+
+int nglod_sample_directions(int64_t n, uint64_t seed, float* out, void* stream) {
+    if (n < 0) return fail(NGLOD_ERR_ARG, "n < 0");
+    if (n == 0) return NGLOD_OK;
+    if (!out) return fail(NGLOD_ERR_ARG, "NULL pointer");
+    hipLaunchKernelGGL(sample_directions_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       seed, n, out);
+    return check_launch("sample_directions_kernel");
 }
 
 int nglod_sample_surface(const float* mesh, const double* cdf, int64_t nf, int64_t n, uint64_t seed, double sigma,

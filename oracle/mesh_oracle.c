@@ -196,6 +196,19 @@ void orc_sample_uniform(uint64_t seed, int64_t n, float* out) {
         for (int k = 0; k < 3; ++k) out[3 * i + k] = (float)(rnd01(seed, (uint64_t)i, (uint32_t)k) * 2.0 - 1.0);
 }
 
+/* Directions uniform on S^2 (sdf-net/lib/geoutils.py:31-40), one stream of
+ * the same counter RNG: draws 0 (z) and 1 (phi). */
+void orc_sample_directions(uint64_t seed, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        const double z = 1.0 - 2.0 * rnd01(seed, (uint64_t)i, 0);
+        const double r = sqrt(1.0 - z * z);
+        const double phi = 2.0 * M_PI * rnd01(seed, (uint64_t)i, 1);
+        out[3 * i] = (float)(r * cos(phi));
+        out[3 * i + 1] = (float)(r * sin(phi));
+        out[3 * i + 2] = (float)z;
+    }
+}
+
 /* Area-weighted surface sample + optional Gaussian offset of std sigma.
  * cdf: inclusive cumulative normalized areas [nf]; face = first f with
  * cdf[f] > r0.  Returns face indices too. */

@@ -282,3 +282,125 @@ def mark_unique_rays(nuggets):
     info = np.empty(n.shape[0], dtype=np.int32)
     lib().orc_mark_unique_rays(_p(n), ctypes.c_int64(n.shape[0]), _p(info))
     return info
+
+
+def sample_directions(seed, n):
+    out = np.empty((n, 3), dtype=np.float32)
+    lib().orc_sample_directions(ctypes.c_uint64(seed), ctypes.c_int64(n), _p(out))
+    return out
+
+
+# ---- sparse (SOL) model -------------------------------------------------------
+def unique_corners(level_points):
+    """geoutils.unique_corners (sdf-net/lib/geoutils.py:60-77): the 8 corners
+    of every cell, deduplicated and sorted by (x, y, z) as torch.unique(dim=0)."""
+    p = np.asarray(level_points)[:, :3].astype(np.int64)
+    offs = np.array([[(j & 4) >> 2, (j & 2) >> 1, j & 1] for j in range(8)], dtype=np.int64)
+    allc = (p[None, :, :] + offs[:, None, :]).reshape(-1, 3)
+    return np.unique(allc, axis=0).astype(np.int32)
+
+
+def codes_from_points(x, level):
+    """Leaf codes of the cells holding x: floor((x+1)/2*res) per axis in FP32
+    (renderutils.py:67-70), clamped to the grid, Morton-coded, sorted, unique."""
+    xf = np.asarray(x, dtype=np.float32).reshape(-1, 3)
+    res = np.float32(1 << level)
+    v = np.floor(((xf + np.float32(1.0)) / np.float32(2.0)) * res)
+    v = np.clip(v, 0, res - 1).astype(np.uint32)
+    codes = np.array([morton(int(a), int(b), int(c)) for a, b, c in v], dtype=np.uint64)
+    return np.unique(codes)
+
+
+class OrcSol(ctypes.Structure):
+    _fields_ = [
+        ("target", ctypes.c_int32),
+        ("lod", ctypes.c_int32),
+        ("din", ctypes.c_int32),
+        ("points", _f),
+        ("psum", _f),
+        ("tv", _f),
+        ("parent", _f),
+        ("cc", _f),
+        ("cf", _f),
+        ("w0", _f),
+        ("b0", _f),
+        ("w1", _f),
+        ("b1", _f),
+    ]
+
+
+def sol_trinkets(points, pyramid, target, cc_blocks):
+    """Per point of levels 2..target: 8 corner indices into the concatenated
+    corner list and the parent point (index_trinket.cuh:31-98)."""
+    pts = np.ascontiguousarray(points, dtype=np.int16)
+    pyr = np.asarray(pyramid, dtype=np.int32)
+    cnt = np.ascontiguousarray(pyr[0], dtype=np.int32)
+    psum = np.ascontiguousarray(pyr[1], dtype=np.int32)
+    cc = np.ascontiguousarray(np.concatenate(cc_blocks, axis=0), dtype=np.int32)
+    sizes = np.array([b.shape[0] for b in cc_blocks], dtype=np.int64)
+    off = np.ascontiguousarray(np.concatenate([[0], np.cumsum(sizes)[:-1]]), dtype=np.int64)
+    sizes = np.ascontiguousarray(sizes)
+    P = pts.shape[0]
+    tv = np.full((P, 8), 0xFFFFFFFF, dtype=np.uint32)
+    parent = np.full(P, -1, dtype=np.int32)
+    lib().orc_sol_trinkets(_p(pts), _p(cnt), _p(psum), ctypes.c_int(target), _p(cc), _p(off), _p(sizes), _p(tv),
+                           _p(parent))
+    return tv, parent, cc
+
+
+class SolArrays:
+    """Host arrays of a sparse model; .struct() keeps them alive."""
+
+    def __init__(self, points, pyramid, target, tv, parent, cc, cf, w0, b0, w1, b1):
+        self.target = int(target)
+        self.lod = self.target - 2
+        self.points = np.ascontiguousarray(points, dtype=np.int16)
+        self.psum = np.ascontiguousarray(np.asarray(pyramid)[1], dtype=np.int32)
+        self.tv = np.ascontiguousarray(tv, dtype=np.uint32)
+        self.parent = np.ascontiguousarray(parent, dtype=np.int32)
+        self.cc = np.ascontiguousarray(cc, dtype=np.int32)
+        self.cf = np.ascontiguousarray(cf, dtype=np.float32)
+        self.w0 = np.ascontiguousarray(w0, dtype=np.float32)
+        self.b0 = np.ascontiguousarray(b0, dtype=np.float32).reshape(-1)
+        self.w1 = np.ascontiguousarray(w1, dtype=np.float32).reshape(-1)
+        self.b1 = np.ascontiguousarray(b1, dtype=np.float32).reshape(-1)
+        self.din = self.w0.shape[1]
+
+    def struct(self):
+        return OrcSol(self.target, self.lod, self.din, _p(self.points), _p(self.psum), _p(self.tv), _p(self.parent),
+                      _p(self.cc), _p(self.cf), _p(self.w0), _p(self.b0), _p(self.w1), _p(self.b1))
+
+
+def sol_eval(sol, x, pidx):
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+    pi = np.ascontiguousarray(pidx, dtype=np.int64)
+    out = np.empty(xx.shape[0], dtype=np.float32)
+    s = sol.struct()
+    lib().orc_sol_eval(ctypes.byref(s), _p(xx), _p(pi), ctypes.c_int64(xx.shape[0]), _p(out))
+    return out
+
+
+def sol_trace(sol, o, d, offsets, nuggets):
+    oo = np.ascontiguousarray(o, dtype=np.float32).reshape(-1, 3)
+    dd = np.ascontiguousarray(d, dtype=np.float32).reshape(-1, 3)
+    offs = np.ascontiguousarray(offsets, dtype=np.int32)
+    nug = np.ascontiguousarray(nuggets, dtype=np.int32).reshape(-1, 2)
+    if nug.size == 0:
+        nug = np.zeros((1, 2), dtype=np.int32)
+    n = oo.shape[0]
+    x = np.empty((n, 3), dtype=np.float32)
+    t = np.empty(n, dtype=np.float32)
+    hit = np.empty(n, dtype=np.uint8)
+    nrm = np.empty((n, 3), dtype=np.float32)
+    it = np.empty(n, dtype=np.int32)
+    s = sol.struct()
+    lib().orc_sol_trace(ctypes.byref(s), _p(oo), _p(dd), ctypes.c_int64(n), _p(offs), _p(nug), _p(x), _p(t), _p(hit),
+                        _p(nrm), _p(it))
+    return {"x": x, "t": t, "hit": hit.astype(bool), "normal": nrm, "iters": it}
+
+
+def nugget_offsets(nuggets, n):
+    """CSR offsets [n+1] of ray-sorted nuggets."""
+    r = np.asarray(nuggets, dtype=np.int64).reshape(-1, 2)[:, 0]
+    counts = np.bincount(r, minlength=n)[:n]
+    return np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)

@@ -44,3 +44,45 @@ def ambiguous_rays(orc, other_steps, num_steps, margin=2e-6, cap_is_ambiguous=Tr
     if cap_is_ambiguous:
         amb |= orc["steps"] >= num_steps
     return amb
+
+
+def sol_arrays(m, octree, lod, fp16=False):
+    """Host sparse model from dense arrays m and octree bytes, as the export
+    builds it (SOL_NGLOD.py:64-70, index_trinket.cuh): corners per level,
+    corner features read from the dense corner-major grids, trinkets."""
+    pts, pyr, L = oracle.octree_points(octree)
+    target = lod + 2
+    assert L >= target
+    blocks = [oracle.unique_corners(pts[pyr[1, l]:pyr[1, l] + pyr[0, l]]) for l in range(2, target + 1)]
+    tv, parent, cc = oracle.sol_trinkets(pts, pyr, target, blocks)
+    cfs = []
+    for l, blk in zip(range(2, target + 1), blocks):
+        i = l - 2
+        S = m.res[i] + 1
+        g = m.grids[i].reshape(S, S, S, -1)
+        cfs.append(g[blk[:, 2], blk[:, 1], blk[:, 0]])
+    cf = np.concatenate(cfs, 0)
+    w0, b0, w1, b1 = m.w0[lod], m.b0[lod], m.w1[lod], m.b1[lod]
+    if fp16:
+        cf, w0, b0, w1, b1 = (a.astype(np.float16).astype(np.float32) for a in (cf, w0, b0, w1, b1))
+    sol = oracle.SolArrays(pts, pyr, target, tv, parent, cc, cf, w0, b0, w1, b1)
+    return sol, blocks, (pts, pyr, L)
+
+
+def level_index(pts, pyr, level, cells):
+    """Index within `level` of integer cells [n, 3] (levels are Morton-sorted);
+    -1 where the cell is not in the octree."""
+    lv = pts[pyr[1, level]:pyr[1, level] + pyr[0, level], :3].astype(np.int64)
+    keys = np.array([oracle.morton(int(a), int(b), int(c)) for a, b, c in lv], dtype=np.uint64)
+    q = np.array([oracle.morton(int(a), int(b), int(c)) for a, b, c in cells], dtype=np.uint64)
+    pos = np.searchsorted(keys, q)
+    pos = np.minimum(pos, len(keys) - 1)
+    return np.where(keys[pos] == q, pos, -1).astype(np.int64)
+
+
+def full_octree(level):
+    """Octree with every cell of `level` occupied."""
+    R = 1 << level
+    g = np.stack(np.meshgrid(np.arange(R), np.arange(R), np.arange(R), indexing="ij"), -1).reshape(-1, 3)
+    codes = np.unique(np.array([oracle.morton(int(a), int(b), int(c)) for a, b, c in g], dtype=np.uint64))
+    return oracle.octree_from_codes(codes, level)[0]
