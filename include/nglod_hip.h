@@ -74,6 +74,14 @@ typedef struct nglod_trace_params {
     double return_dist;   /* NGLOD_MIXED only: an FP32 step that grows past this
                              (the ray leaves the surface: a near miss) returns
                              the ray to mixed precision, at most twice */
+    float* pass_ms;       /* optional host float[4] (NULL: off).  When set the
+                             call records HIP events between its launches,
+                             waits for the last one (the call becomes
+                             synchronous) and writes milliseconds per pass:
+                             [0] the march kernel (FP32 form) or the mixed
+                             march, [1] the FP32 continuation (mixed form),
+                             [2] the normals pass (mixed form), [3] counter
+                             resets.  For measurement, not for serving. */
 } nglod_trace_params_t;
 
 int nglod_abi_version(void);

@@ -78,9 +78,53 @@ inline int decoder_of(const nglod_model_t* m, int lod) { return m->num_decoders 
 
 int device_cu_count();
 
+// Optional host-side timing of a launch sequence: mark(slot) records an event
+// and charges the time since the previous mark to out[slot].  Inactive (no
+// events, no sync) when out is NULL; finish() synchronizes on the last event.
+class PassTimer {
+  public:
+    static constexpr int kMax = 32;
+    PassTimer(float* out, int nslots, hipStream_t s) : out_(out), nslots_(nslots), s_(s) {
+        if (!out_) return;
+        for (int i = 0; i < nslots_; ++i) out_[i] = 0.0f;
+        for (int i = 0; i < kMax; ++i) ev_[i] = nullptr;
+        mark(-1);
+    }
+    ~PassTimer() {
+        for (int i = 0; i < n_; ++i) hipEventDestroy(ev_[i]);
+    }
+    void mark(int slot) {
+        if (!out_ || n_ >= kMax) return;
+        if (hipEventCreate(&ev_[n_]) != hipSuccess) return;
+        hipEventRecord(ev_[n_], s_);
+        slot_[n_++] = slot;
+    }
+    int finish() {
+        if (!out_ || n_ == 0) return NGLOD_OK;
+        if (hipEventSynchronize(ev_[n_ - 1]) != hipSuccess) return fail(NGLOD_ERR_HIP, "timing sync failed");
+        for (int i = 1; i < n_; ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev_[i - 1], ev_[i]) != hipSuccess) return fail(NGLOD_ERR_HIP, "timing failed");
+            if (slot_[i] >= 0 && slot_[i] < nslots_) out_[slot_[i]] += ms;
+        }
+        return NGLOD_OK;
+    }
+
+  private:
+    float* out_;
+    int nslots_;
+    hipStream_t s_;
+    hipEvent_t ev_[kMax];
+    int slot_[kMax];
+    int n_ = 0;
+};
+
+// Slots of nglod_trace_params_t::pass_ms.
+enum : int { kPassMarch = 0, kPassFp32 = 1, kPassNormals = 2, kPassOther = 3, kPassSlots = 4 };
+
 struct TraceArgs;
 // Mixed-precision launchers (mixed.hip).
 int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out, hipStream_t s);
-int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s);
+int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& timer);
 
 }  // namespace nglod

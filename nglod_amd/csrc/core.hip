@@ -343,10 +343,15 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     a.evals = (unsigned long long*)evals_out;
     a.ray_steps = ray_steps;
     split_queues(n, a.qbeg, a.qend);
+    PassTimer timer(params->pass_ms, kPassSlots, s);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, a.queue, kQueues + 5, 0u);
     rc = check_launch("fill_u32_kernel");
     if (rc) return rc;
-    if (form == NGLOD_MIXED) return launch_trace_mixed(a, n, s);
+    timer.mark(kPassOther);
+    if (form == NGLOD_MIXED) {
+        rc = launch_trace_mixed(a, n, s, timer);
+        return rc ? rc : timer.finish();
+    }
     if (m.pos_inv) {
         const int grid = persistent_grid(trace_dense_fp32_kernel<true>, n, 256);
         hipLaunchKernelGGL(trace_dense_fp32_kernel<true>, dim3(grid), dim3(256), 0, s, a);
@@ -354,7 +359,10 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
         const int grid = persistent_grid(trace_dense_fp32_kernel<false>, n, 256);
         hipLaunchKernelGGL(trace_dense_fp32_kernel<false>, dim3(grid), dim3(256), 0, s, a);
     }
-    return check_launch("trace_dense_fp32_kernel");
+    rc = check_launch("trace_dense_fp32_kernel");
+    if (rc) return rc;
+    timer.mark(kPassMarch);
+    return timer.finish();
 }
 
 }  // extern "C"

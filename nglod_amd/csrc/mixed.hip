@@ -308,7 +308,7 @@ __global__ void reset_counters_kernel(uint32_t* c, int which) {
 constexpr int kMixedRounds = 3;
 
 template <bool POSINV>
-static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s) {
+static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& timer) {
     const int g1 = grid_for(trace_mixed_pass1_kernel<POSINV>, n, 256);
     const int g2 = grid_for(trace_mixed_pass2_kernel<POSINV>, n, 256);
     for (int r = 0; r < kMixedRounds; ++r) {
@@ -317,26 +317,33 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s) {
             hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, a.queue, 0);
             int rc = check_launch("reset_counters_kernel");
             if (rc) return rc;
+            timer.mark(kPassOther);
         }
         a.replay = 1;
         hipLaunchKernelGGL(trace_mixed_pass1_kernel<POSINV>, dim3(g1), dim3(256), 0, s, a);
         a.replay = 0;
         int rc = check_launch("trace_mixed_pass1_kernel");
         if (rc) return rc;
+        timer.mark(kPassMarch);
         hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, a.queue, 1);
         rc = check_launch("reset_counters_kernel");
         if (rc) return rc;
+        timer.mark(kPassOther);
         a.allow_return = r + 1 < kMixedRounds;
         hipLaunchKernelGGL(trace_mixed_pass2_kernel<POSINV>, dim3(g2), dim3(256), 0, s, a);
         rc = check_launch("trace_mixed_pass2_kernel");
         if (rc) return rc;
+        timer.mark(kPassFp32);
     }
     hipLaunchKernelGGL(normals_list_kernel<POSINV>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
-    return check_launch("normals_list_kernel");
+    int rc = check_launch("normals_list_kernel");
+    if (rc) return rc;
+    timer.mark(kPassNormals);
+    return NGLOD_OK;
 }
 
-int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s) {
-    return a.m.pos_inv ? trace_mixed_rounds<true>(a, n, s) : trace_mixed_rounds<false>(a, n, s);
+int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& timer) {
+    return a.m.pos_inv ? trace_mixed_rounds<true>(a, n, s, timer) : trace_mixed_rounds<false>(a, n, s, timer);
 }
 
 }  // namespace nglod

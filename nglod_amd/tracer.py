@@ -6,6 +6,7 @@ sdf-net/lib/tracer/{BaseTracer,SphereTracer,RenderBuffer}.py.
 persistent HIP kernel (nglod_trace_dense) instead of a Python loop with a
 host sync and a mask compaction per step.
 """
+import ctypes
 from dataclasses import asdict, astuple, dataclass, fields
 from typing import List, Union
 
@@ -156,6 +157,10 @@ class SphereTracer(BaseTracer):
         self.return_dist = return_dist
         self._ws = {}
         self.last_evals = None
+        # When True, each call times its passes with HIP events (and waits
+        # for them): last_pass_ms = [march, FP32 continuation, normals, resets].
+        self.time_passes = False
+        self.last_pass_ms = None
 
     def _workspace(self, device, n, form):
         key = (device.type, device.index)
@@ -194,6 +199,8 @@ class SphereTracer(BaseTracer):
         p.normals = 1
         p.refine_steps = int(self.refine_steps)
         p.return_dist = float(self.return_dist)
+        pass_ms = (ctypes.c_float * 4)() if self.time_passes else None
+        p.pass_ms = ctypes.cast(pass_ms, ctypes.c_void_p) if pass_ms is not None else None
         m = net.descriptor(form)
         ws = self._workspace(dev, n, form)
         lib = _lib.load()
@@ -201,4 +208,5 @@ class SphereTracer(BaseTracer):
                                          _lib.ptr(hit), _lib.ptr(normal), _lib.ptr(ws), ws.numel(), _lib.ptr(evals),
                                          _lib.ptr(ray_steps), _lib.stream_of(o)), "nglod_trace_dense")
         self.last_evals = evals
+        self.last_pass_ms = list(pass_ms) if pass_ms is not None else None
         return RenderBuffer(x=x, depth=t, hit=hit, normal=normal)
