@@ -67,6 +67,35 @@ def cpu_baseline(net, o, d, lod, num_steps, sample_rays, threads):
                       f"{threads} threads, {cpu_model()}, {dt:.1f} s"}
 
 
+def serve_bench(args, dev):
+    """Mixed-request stream over 8 resident models (seeds 0-7: displaced
+    icospheres, each fitted args.serve_fit_iters Adam steps): saturation
+    throughput with every request at t = 0, then Poisson arrivals at 0.3, 0.6
+    and 0.9 of the saturated request rate.  Resolutions uniform in {64^2,
+    256^2, 640x480, 1280x720, 1920x1080}, LOD uniform 0..4, orbit cameras."""
+    import math
+
+    import torch
+
+    from nglod_amd import trainer
+    from nglod_amd.serve import TraceServer, request_stream
+
+    models = [trainer.fitted_icosphere_model(seed=s, device=dev, iterations=args.serve_fit_iters)[0]
+              for s in range(8)]
+    server = TraceServer(models, pool_bytes=2 << 30, slots=4, form="mixed")
+    server.run(request_stream(8, math.inf, 8, seed=99))  # warm-up
+    sat = server.run(request_stream(args.serve_requests, math.inf, 8, seed=100)).summary()
+    req_rate = sat["requests"] / sat["wall_s"]
+    out = {"models": 8, "slots": 4, "pool_bytes": server.pool.nbytes, "form": "mixed",
+           "saturation": {**sat, "requests_per_s": req_rate}, "rates": {}}
+    for k, frac in enumerate((0.3, 0.6, 0.9)):
+        st = server.run(request_stream(args.serve_requests, frac * req_rate, 8, seed=101 + k)).summary()
+        out["rates"][f"{frac:.1f}x"] = {**st, "arrival_rate_per_s": frac * req_rate}
+    del server, models
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +108,9 @@ def main():
     ap.add_argument("--num-steps", type=int, default=128)
     ap.add_argument("--cpu-sample", type=int, default=65536, help="rays in the CPU baseline sample (0 = skip)")
     ap.add_argument("--fit-iters", type=int, default=200, help="Adam steps of the seeded model fit")
+    ap.add_argument("--serve-requests", type=int, default=100,
+                    help="requests per arrival rate in the serving run (0 = skip)")
+    ap.add_argument("--serve-fit-iters", type=int, default=50, help="Adam steps per served model")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -202,6 +234,8 @@ def main():
                       "achieved_GBs": r["achieved_GBs"]},
         },
     }
+    if args.serve_requests > 0:
+        line["serving"] = serve_bench(args, dev)
     if args.cpu_sample > 0:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(net, o, d, args.lod, args.num_steps, args.cpu_sample, threads)

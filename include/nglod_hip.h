@@ -262,6 +262,13 @@ int nglod_sample_surface(const float* mesh, const double* cdf, int64_t nf, int64
  * counter-based stream as nglod_sample_uniform. */
 int nglod_sample_directions(int64_t n, uint64_t seed, float* out, void* stream);
 
+/* Camera rays of a look-at camera, width*height rays, ray index =
+ * ix * height + iy (geoutils.look_at + normalized_grid,
+ * sdf-net/lib/geoutils.py:140-206, jitter off; world up (0, 1, 0)).  eye, at:
+ * host float[3]; ortho = 0 for the perspective camera.  ray_o, ray_d [N][3]. */
+int nglod_camera_rays(const float* eye, const float* at, float fov_deg, int32_t width, int32_t height,
+                      int32_t ortho, float* ray_o, float* ray_d, void* stream);
+
 /* ---- sparse ("SOL") model and its march ---------------------------------- */
 
 /* Sparse model: features live only on the corners of occupied octree cells.

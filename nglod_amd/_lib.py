@@ -132,6 +132,7 @@ SIGNATURES = {
     "nglod_sample_uniform": (ctypes.c_int, [_i64, ctypes.c_uint64, _p, _p]),
     "nglod_sample_surface": (ctypes.c_int, [_p, _p, _i64, _i64, ctypes.c_uint64, ctypes.c_double, _p, _p, _p]),
     "nglod_sample_directions": (ctypes.c_int, [_i64, ctypes.c_uint64, _p, _p]),
+    "nglod_camera_rays": (ctypes.c_int, [_p, _p, ctypes.c_float, _i32, _i32, _i32, _p, _p, _p]),
     "nglod_sol_corners": (ctypes.c_int, [_p, _p, _i32, _i32, _p, _i64, _p, _p, _sz, ctypes.POINTER(_sz), _p]),
     "nglod_sol_features": (ctypes.c_int, [ctypes.POINTER(NglodModel), _i32, _p, _p, _p, _p]),
     "nglod_sol_trinkets": (ctypes.c_int, [_p, _p, _i32, _i32, _p, _p, _p, _p, _p, _sz, _p]),
