@@ -67,6 +67,45 @@ def cpu_baseline(net, o, d, lod, num_steps, sample_rays, threads):
                       f"{threads} threads, {cpu_model()}, {dt:.1f} s"}
 
 
+def sparse_frame(args, net, o, d, dev):
+    """The sparse path on the same frame: the fitted model exported to the
+    sparse format (FP16 corner features and decoders, as the export stores
+    them) over the level-7 octree of the displaced icosphere (exact
+    triangle-cell overlap), nuggets at level lod+2, then the sparse march with
+    normals.  Times both launches with HIP events."""
+    import torch
+
+    from nglod_amd import mesh
+    from nglod_amd.sol import SolNGLOD
+    from nglod_amd.spc import SPC
+
+    V, F = mesh.displaced_icosphere(0)
+    spc = SPC.from_mesh(mesh.triangles(V, F), 7, dev)
+    sol = SolNGLOD.from_net(net, spc, fp16=True)
+    target = args.lod + 2
+    for _ in range(args.warmup):
+        sol.trace(o, d, lod=args.lod)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t_nug = t_march = 0.0
+    for _ in range(args.steps):
+        ev[0].record()
+        nug = spc.raytrace(o, d, target)
+        ev[1].record()
+        rb = sol.trace(o, d, lod=args.lod, nuggets=nug)
+        ev[2].record()
+        torch.cuda.synchronize()
+        t_nug += ev[0].elapsed_time(ev[1])
+        t_march += ev[1].elapsed_time(ev[2])
+    n = o.shape[0]
+    ms = (t_nug + t_march) / args.steps
+    return {"rays_per_s": n / (ms * 1e-3), "ms_per_frame": ms, "raytrace_ms": t_nug / args.steps,
+            "march_ms": t_march / args.steps, "nuggets": int(nug[0].shape[0]), "octree_level": 7,
+            "leaves": int(spc.pyramid[0, 7]), "hits": int(rb.hit.sum().item()),
+            "sdf_evals": int(sol.last_evals.item()),
+            "note": "raytrace_ms includes the host read-back of the nugget count (a setup-style synchronous call)"}
+
+
 def serve_bench(args, dev):
     """Mixed-request stream over 8 resident models (seeds 0-7: displaced
     icospheres, each fitted args.serve_fit_iters Adam steps): saturation
@@ -108,6 +147,7 @@ def main():
     ap.add_argument("--num-steps", type=int, default=128)
     ap.add_argument("--cpu-sample", type=int, default=65536, help="rays in the CPU baseline sample (0 = skip)")
     ap.add_argument("--fit-iters", type=int, default=200, help="Adam steps of the seeded model fit")
+    ap.add_argument("--sparse", type=int, default=1, help="also time the sparse (octree) path on the frame")
     ap.add_argument("--serve-requests", type=int, default=100,
                     help="requests per arrival rate in the serving run (0 = skip)")
     ap.add_argument("--serve-fit-iters", type=int, default=50, help="Adam steps per served model")
@@ -189,6 +229,7 @@ def main():
 
     forms = ["fp32", "mixed"] if args.form == "both" else [args.form]
     res = {f: run_form(f) for f in forms}
+    sparse = sparse_frame(args, net, o, d, dev) if args.sparse else None
     primary = "mixed" if "mixed" in res else "fp32"
     r = res[primary]
     # the dominant kernel of the primary form's frame
@@ -217,6 +258,7 @@ def main():
             "lod": args.lod,
             "form": primary,
             "forms": res,
+            "sparse": sparse,
             "model_fit_s": t_fit,
         },
         "roofline": {
