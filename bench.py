@@ -67,6 +67,20 @@ def cpu_baseline(net, o, d, lod, num_steps, sample_rays, threads):
                       f"{threads} threads, {cpu_model()}, {dt:.1f} s"}
 
 
+def copy_probe(dev):
+    """Five device-to-device copies of 1 GiB by torch's copy kernel: reads and
+    writes exactly 1 GiB each, so FETCH_SIZE / WRITE_SIZE per dispatch of this
+    kernel give the counters' scale for HBM bytes."""
+    import torch
+
+    src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": src.numel() * 4, "copies": 5}), flush=True)
+
+
 def sparse_frame(args, net, o, d, dev):
     """The sparse path on the same frame: the fitted model exported to the
     sparse format (FP16 corner features and decoders, as the export stores
@@ -148,6 +162,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=65536, help="rays in the CPU baseline sample (0 = skip)")
     ap.add_argument("--fit-iters", type=int, default=200, help="Adam steps of the seeded model fit")
     ap.add_argument("--sparse", type=int, default=1, help="also time the sparse (octree) path on the frame")
+    ap.add_argument("--model-cache", default="", help="load the fitted model from / save it to this file")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run 5 copies of 1 GiB (the counter calibration for HBM traffic) and exit")
     ap.add_argument("--serve-requests", type=int, default=100,
                     help="requests per arrival rate in the serving run (0 = skip)")
     ap.add_argument("--serve-fit-iters", type=int, default=50, help="Adam steps per served model")
@@ -163,8 +180,17 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.synchronize()
+    if args.copy_probe:
+        copy_probe(dev)
+        return
     t_fit = time.perf_counter()
-    net, md = trainer.fitted_icosphere_model(seed=0, device=dev, iterations=args.fit_iters)
+    if args.model_cache and os.path.exists(args.model_cache):
+        net = workloads.make_model(num_lods=5, pos_invariant=True, seed=0, device=dev)
+        net.load_state_dict(torch.load(args.model_cache, map_location=dev, weights_only=True))
+    else:
+        net, md = trainer.fitted_icosphere_model(seed=0, device=dev, iterations=args.fit_iters)
+        if args.model_cache:
+            torch.save(net.state_dict(), args.model_cache)
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t_fit
     net.lod = args.lod

@@ -22,10 +22,15 @@ NPZ_KEYS = ("octree", "cc", "cf", "w0", "b0", "w1", "b1", "pyramid")
 def voxel_sparsify(n, net, lod, seed=0, max_batches=1000, device=None):
     """Level-(lod+2) cells that hold traced surface points
     (renderutils.voxel_sparsify / sample_surface, sdf-net/lib/renderutils.py:30-73):
-    batches of n rays with origins uniform in [-1,1]^3 and directions uniform on
-    the sphere, traced at `lod`, until at least n rays hit.  Every hit of the
-    traced batches is kept (the reference keeps the first n).  Returns the SPC.
-    Rays come from seeded counter streams, so the set is reproducible."""
+    batches of n random lines through the volume (a point uniform in [-1,1]^3
+    and a direction uniform on the sphere), traced at `lod`, until at least n
+    rays hit.  Each ray starts 4 units back along its line, outside the box:
+    the tracer reports an origin inside the box as a hit at depth 0 (the box
+    entry skips such rays, SphereTracer.py:53 with sol_nglod_kernel.cu:109-112),
+    which would fill every cell.  A hit counts when |sdf(x)| is below half a
+    cell (rays stopped by the step cap are not surface points).  Every such hit
+    of the traced batches is kept (the reference keeps the first n).  Rays come
+    from seeded counter streams, so the set is reproducible."""
     dev = torch.device(device) if device is not None else net.features[0].fm.device
     lib = _lib.load()
     level = lod + 2
@@ -42,10 +47,13 @@ def voxel_sparsify(n, net, lod, seed=0, max_batches=1000, device=None):
                        "nglod_sample_uniform")
             _lib.check(lib.nglod_sample_directions(n, 2 * (seed * max_batches + b) + 1, _lib.ptr(d), s),
                        "nglod_sample_directions")
+            o.sub_(4.0 * d)
             rb = tracer(net, o, d)
+            near = net.sdf_hip(rb.x, lod)[:, 0].abs() < 1.0 / (1 << level)
+            keep = rb.hit & near
             xs.append(rb.x)
-            masks.append(rb.hit)
-            hits += int(rb.hit.sum().item())
+            masks.append(keep)
+            hits += int(keep.sum().item())
             if hits >= n:
                 break
     finally:
