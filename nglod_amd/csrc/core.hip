@@ -8,6 +8,7 @@
 #include <stdio.h>
 
 #include "api_common.hpp"
+#include "fp32_wave.hpp"
 #include "trace_common.hpp"
 
 namespace nglod {
@@ -83,9 +84,17 @@ __global__ __launch_bounds__(256) void grid_sample_kernel(const float* __restric
 template <bool POSINV>
 __global__ __launch_bounds__(256) void sdf_fp32_kernel(ModelArgs m, int lod, int dec, const float* __restrict__ x,
                                                        int64_t n, float* __restrict__ out) {
+    __shared__ Fp32Lds lds;
+    stage_fp32_lds(m, dec, &lds);
+    __syncthreads();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[i] = sdf_fp32<POSINV>(m, lod, dec, x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    const bool valid = i < n;
+    if (__ballot(valid) == 0) return;  // whole wave past the end (no barrier follows)
+    const float px = valid ? x[3 * i] : 0.0f;
+    const float py = valid ? x[3 * i + 1] : 0.0f;
+    const float pz = valid ? x[3 * i + 2] : 0.0f;
+    const float d = sdf_fp32_wave<POSINV>(m, lod, &lds, m.b1[dec][0], lane_id(), valid, px, py, pz);
+    if (valid) out[i] = d;
 }
 
 template <bool POSINV>
@@ -141,6 +150,10 @@ __global__ __launch_bounds__(256) void normals_fp32_kernel(ModelArgs m, int lod,
 
 template <bool POSINV>
 __global__ __launch_bounds__(256) void trace_dense_fp32_kernel(TraceArgs a) {
+    __shared__ Fp32Lds lds;
+    stage_fp32_lds(a.m, a.dec, &lds);
+    __syncthreads();
+    const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
     int q = blockIdx.x % kQueues;
     int tried = 0;
@@ -171,8 +184,9 @@ __global__ __launch_bounds__(256) void trace_dense_fp32_kernel(TraceArgs a) {
             }
         }
         if (__ballot(s.phase != PH_IDLE) == 0) break;
-        if (s.phase != PH_IDLE) {
-            const float d = sdf_fp32<POSINV>(a.m, a.lod, a.dec, s.qx, s.qy, s.qz);
+        const bool busy = s.phase != PH_IDLE;
+        const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        if (busy) {
             ++nevals;
             s.consume(a, d);
         }

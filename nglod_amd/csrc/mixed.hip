@@ -15,6 +15,7 @@
 //             kMixedRounds times; the last pass 2 keeps every ray.
 //     pass 3  FP32 normals of every finished hit, one hit per lane.
 #include "api_common.hpp"
+#include "fp32_wave.hpp"
 #include "mixed_device.hpp"
 #include "trace_common.hpp"
 
@@ -54,7 +55,8 @@ __global__ __launch_bounds__(256) void sdf_mixed_kernel(ModelArgs m, int lod, in
     const float px = valid ? x[3 * i] : 0.0f;
     const float py = valid ? x[3 * i + 1] : 0.0f;
     const float pz = valid ? x[3 * i + 2] : 0.0f;
-    const float d = sdf_mixed_wave<POSINV>(m, lod, &lds, b1, lane, px, py, pz);
+    if (__ballot(valid) == 0) return;  // whole wave past the end (no barrier follows)
+    const float d = sdf_mixed_wave<POSINV>(m, lod, &lds, b1, lane, valid, px, py, pz);
     if (valid) out[i] = d;
 }
 
@@ -141,8 +143,7 @@ __global__ __launch_bounds__(256) void trace_mixed_pass1_kernel(TraceArgs a) {
         }
         if (__ballot(s.phase != PH_IDLE) == 0) break;
         const bool busy = s.phase != PH_IDLE;
-        const float d = sdf_mixed_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy ? s.qx : 0.0f,
-                                               busy ? s.qy : 0.0f, busy ? s.qz : 0.0f);
+        const float d = sdf_mixed_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
             // pre-evaluation state of the last kReplay evaluations (shift register)
@@ -197,6 +198,10 @@ __global__ __launch_bounds__(256) void trace_mixed_pass1_kernel(TraceArgs a) {
 
 template <bool POSINV>
 __global__ __launch_bounds__(256) void trace_mixed_pass2_kernel(TraceArgs a) {
+    __shared__ Fp32Lds lds;
+    stage_fp32_lds(a.m, a.dec, &lds);
+    __syncthreads();
+    const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
     const uint32_t total = *a.handoff_count;
     bool drained = false;
@@ -222,8 +227,9 @@ __global__ __launch_bounds__(256) void trace_mixed_pass2_kernel(TraceArgs a) {
             }
         }
         if (__ballot(s.phase != PH_IDLE) == 0) break;
-        if (s.phase != PH_IDLE) {
-            const float d = sdf_fp32<POSINV>(a.m, a.lod, a.dec, s.qx, s.qy, s.qz);
+        const bool busy = s.phase != PH_IDLE;
+        const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        if (busy) {
             ++nevals;
             s.consume(a, d);
         }
@@ -249,21 +255,30 @@ __global__ __launch_bounds__(256) void trace_mixed_pass2_kernel(TraceArgs a) {
 template <bool POSINV>
 __global__ __launch_bounds__(256) void normals_list_kernel(TraceArgs a) {
     const uint32_t total = *a.ncount;
+    if ((int64_t)blockIdx.x * blockDim.x >= (int64_t)total) return;  // block-uniform, before the barrier
+    __shared__ Fp32Lds lds;
+    stage_fp32_lds(a.m, a.dec, &lds);
+    __syncthreads();
+    const float b1 = a.m.b1[a.dec][0];
+    const uint32_t lane = lane_id();
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < total;
+    if (__ballot(valid) == 0) return;  // whole wave past the end (no barrier follows)
     uint32_t nevals = 0;
-    if (i < total) {
-        const NormalItem it = a.nlist[i];
-        const float p[3] = {it.x0, it.x1, it.x2};
-        float g[3];
+    NormalItem it = {0, 0.0f, 0.0f, 0.0f};
+    if (valid) it = a.nlist[i];
+    const float p[3] = {it.x0, it.x1, it.x2};
+    float g[3];
 #pragma unroll 1
-        for (int ax = 0; ax < 3; ++ax) {
-            float pp[3] = {p[0] + 0.0f, p[1] + 0.0f, p[2] + 0.0f}, pm[3] = {p[0] - 0.0f, p[1] - 0.0f, p[2] - 0.0f};
-            pp[ax] = p[ax] + kFdH;
-            pm[ax] = p[ax] - kFdH;
-            const float fp = sdf_fp32<POSINV>(a.m, a.lod, a.dec, pp[0], pp[1], pp[2]);
-            const float fm = sdf_fp32<POSINV>(a.m, a.lod, a.dec, pm[0], pm[1], pm[2]);
-            g[ax] = (fp - fm) / kFdTwoH;
-        }
+    for (int ax = 0; ax < 3; ++ax) {
+        float pp[3] = {p[0] + 0.0f, p[1] + 0.0f, p[2] + 0.0f}, pm[3] = {p[0] - 0.0f, p[1] - 0.0f, p[2] - 0.0f};
+        pp[ax] = p[ax] + kFdH;
+        pm[ax] = p[ax] - kFdH;
+        const float fp = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pp[0], pp[1], pp[2]);
+        const float fm = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pm[0], pm[1], pm[2]);
+        g[ax] = (fp - fm) / kFdTwoH;
+    }
+    if (valid) {
         nevals = 6;
         normalize_grad(g[0], g[1], g[2], a.normal[3 * it.ray], a.normal[3 * it.ray + 1], a.normal[3 * it.ray + 2]);
     }

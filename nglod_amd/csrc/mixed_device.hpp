@@ -129,12 +129,21 @@ __device__ __forceinline__ float layer2_partial(const f32x16& c, int h, uint32_t
 
 // SDF of the 64 lanes' query points.  Every lane of the wave must call this
 // (no divergent branch around it): the MFMAs and lane swaps are wave-wide.
-// Lanes without a query pass any point in [-1,1]^3 and ignore the result.
+// Lanes without a query pass active = false: they load nothing, and a
+// 32-query tile with no active lane skips its MFMAs.  Each output column
+// depends only on its own query.
 template <bool POSINV>
 __device__ __forceinline__ float sdf_mixed_wave(const ModelArgs& m, int lod, const MixedLds* s, float b1,
-                                                uint32_t lane, float px, float py, float pz) {
+                                                uint32_t lane, bool active, float px, float py, float pz) {
     __half2 acc[kC / 2];
-    gather_fp16(m, lod, px, py, pz, acc);
+    if (active) {
+        gather_fp16(m, lod, px, py, pz, acc);
+    } else {
+#pragma unroll
+        for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
+    }
+    const uint64_t am = __ballot(active);
+    const bool use0 = (uint32_t)am != 0u, use1 = (uint32_t)(am >> 32) != 0u;
     const uint32_t half = lane >> 5;
     const bool lo = half == 0;
     // own channels 16kb + 0..7 (Y) and 16kb + 8..15 (X) as bf16
@@ -173,13 +182,18 @@ __device__ __forceinline__ float sdf_mixed_wave(const ModelArgs& m, int lod, con
     for (int h = 0; h < 4; ++h) {
         const bf16x8 a0 = __builtin_bit_cast(bf16x8, s->wfrag[(h * 2 + 0) * 64 + lane]);
         const bf16x8 a1 = __builtin_bit_cast(bf16x8, s->wfrag[(h * 2 + 1) * 64 + lane]);
-        f32x16 c0 = {}, c1 = {};
-        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B0[0], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B1[0], c1, 0, 0, 0);
-        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, B0[1], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, B1[1], c1, 0, 0, 0);
-        part0 += layer2_partial(c0, h, half, s, !POSINV, qx0, qy0, qz0);
-        part1 += layer2_partial(c1, h, half, s, !POSINV, qx1, qy1, qz1);
+        if (use0) {
+            f32x16 c0 = {};
+            c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B0[0], c0, 0, 0, 0);
+            c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, B0[1], c0, 0, 0, 0);
+            part0 += layer2_partial(c0, h, half, s, !POSINV, qx0, qy0, qz0);
+        }
+        if (use1) {
+            f32x16 c1 = {};
+            c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B1[0], c1, 0, 0, 0);
+            c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, B1[1], c1, 0, 0, 0);
+            part1 += layer2_partial(c1, h, half, s, !POSINV, qx1, qy1, qz1);
+        }
     }
     part0 += __shfl_xor(part0, 32);
     part1 += __shfl_xor(part1, 32);
