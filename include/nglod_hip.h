@@ -74,6 +74,12 @@ typedef struct nglod_trace_params {
     double return_dist;   /* NGLOD_MIXED only: an FP32 step that grows past this
                              (the ray leaves the surface: a near miss) returns
                              the ray to mixed precision, at most twice */
+    double near_dist;     /* NGLOD_MIXED only: of the last refine_steps
+                             evaluations, the stopping one and every earlier
+                             one back to the first step of at least near_dist
+                             (a step taken far from the surface, exact enough
+                             in mixed precision) are re-evaluated; 0 redoes
+                             all refine_steps */
     float* pass_ms;       /* optional host float[4] (NULL: off).  When set the
                              call records HIP events between its launches,
                              waits for the last one (the call becomes

@@ -68,6 +68,7 @@ class NglodTraceParams(ctypes.Structure):
         ("normals", _i32),
         ("refine_steps", _i32),
         ("return_dist", ctypes.c_double),
+        ("near_dist", ctypes.c_double),
         ("pass_ms", _p),
     ]
 

@@ -316,6 +316,7 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     if (params->num_steps < 1) return fail(NGLOD_ERR_ARG, "num_steps must be >= 1");
     if (form == NGLOD_MIXED && (params->refine_steps < 1 || params->refine_steps > 4))
         return fail(NGLOD_ERR_ARG, "refine_steps must be 1..4");
+    if (form == NGLOD_MIXED && !(params->near_dist >= 0.0)) return fail(NGLOD_ERR_ARG, "near_dist must be >= 0");
     if (n < 0 || n > (int64_t)0x7fff0000) return fail(NGLOD_ERR_ARG, "n out of range");
     if (n == 0) return NGLOD_OK;
     if (!ray_o || !ray_d || !x || !t || !hit || !normal) return fail(NGLOD_ERR_ARG, "NULL pointer");
@@ -354,6 +355,7 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     a.defer_normals = form == NGLOD_MIXED ? 1 : 0;
     a.replay = 0;
     a.replay_steps = params->refine_steps < 1 ? 1 : params->refine_steps;
+    a.near_dist = (float)params->near_dist;
     a.evals = (unsigned long long*)evals_out;
     a.ray_steps = ray_steps;
     split_queues(n, a.qbeg, a.qend);

@@ -5,7 +5,8 @@
 //   launch_trace_mixed      nglod_trace_dense(form = NGLOD_MIXED):
 //     pass 1  persistent mixed march.  When a ray stops on a surface test
 //             (|d| <= min_dis or the oscillation test) it rewinds its last
-//             replay_steps evaluations and is appended to a hand-off list
+//             near-surface evaluations (at most replay_steps, back to the
+//             last step of at least near_dist) and is appended to a hand-off list
 //             (ballot + one atomic per wave); finished hits go to the
 //             normal list.
 //     pass 2  persistent FP32 kernel over the hand-off list: re-evaluates the
@@ -156,9 +157,21 @@ __global__ __launch_bounds__(256) void trace_mixed_pass1_kernel(TraceArgs a) {
             ++hist_n;
             s.consume(a, d);
             if (s.phase == PH_REPLAY) {
-                // Rewind j evaluations: the oldest of the last kReplay, or the
-                // box-entry evaluation if the ray made fewer steps.
-                const int j = hist_n < a.replay_steps ? hist_n : a.replay_steps;
+                // Rewind j evaluations (at most replay_steps, back to the
+                // box-entry evaluation if the ray made fewer steps): the
+                // stopping one and those before it back to (not including)
+                // the last step of at least near_dist.  A step that long was
+                // taken away from the surface, where the mixed-precision
+                // error cannot carry the ray past it.
+                const int jmax = hist_n < a.replay_steps ? hist_n : a.replay_steps;
+                int j = 1;
+                if (a.near_dist <= 0.0f) {
+                    j = jmax;
+                } else {
+                    if (j < jmax && fabsf(hd0) < a.near_dist) ++j;
+                    if (j == 2 && j < jmax && fabsf(hd1) < a.near_dist) ++j;
+                    if (j == 3 && j < jmax && fabsf(hd2) < a.near_dist) ++j;
+                }
                 const float tj = j == 1 ? ht0 : j == 2 ? ht1 : j == 3 ? ht2 : ht3;
                 const float dj = j == 1 ? hd0 : j == 2 ? hd1 : j == 3 ? hd2 : hd3;
                 if (hist_init && j == hist_n) {

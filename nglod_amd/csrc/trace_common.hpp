@@ -73,7 +73,8 @@ struct TraceArgs {
     int32_t* ray_steps;
     // Mixed form: rays handed from the mixed march to the FP32 pass.
     int replay;        // 1 in the mixed march: a surface stop becomes PH_REPLAY
-    int replay_steps;  // evaluations rewound and re-evaluated in FP32
+    int replay_steps;  // at most this many evaluations rewound and re-evaluated in FP32
+    float near_dist;   // ... back to the last step of at least this length (0: all replay_steps)
     struct Handoff* handoff;
     uint32_t* handoff_count;
     uint32_t* handoff_queue;

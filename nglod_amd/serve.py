@@ -144,7 +144,8 @@ class TraceServer:
     priority); pool_bytes: the ray-pool capacity that caps admission."""
 
     def __init__(self, models, pool_bytes=1 << 30, slots=4, form="mixed", num_steps=128, min_dis=3e-4,
-                 far_clip=10.0, step_size=1.0, refine_steps=4, return_dist=1e-2, keep_outputs=False):
+                 far_clip=10.0, step_size=1.0, refine_steps=4, return_dist=1e-2, near_dist=1e-2,
+                 keep_outputs=False):
         self.models = list(models)
         self.device = self.models[0].features[0].fm.device
         self.form = _lib.MIXED if form == "mixed" else _lib.FP32
@@ -160,6 +161,7 @@ class TraceServer:
         p.normals = 1
         p.refine_steps = int(refine_steps)
         p.return_dist = float(return_dist)
+        p.near_dist = float(near_dist)
         self.params = p
         self.lib = _lib.load()
 

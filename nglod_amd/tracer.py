@@ -143,18 +143,21 @@ class SphereTracer(BaseTracer):
     (sdf-net/lib/tracer/SphereTracer.py:41-132).
 
     form: 'fp32' (the reference's arithmetic) or 'mixed' (FP16 grids, BF16
-    MFMA decoder; when the march stops on a surface test its last
-    `refine_steps` evaluations are redone in FP32 and it goes on in FP32 to
-    the FP32 stop; normals are FP32).
+    MFMA decoder; when the march stops on a surface test its last near-surface
+    evaluations -- at most `refine_steps`, back to the last step of at least
+    `near_dist`; near_dist=0 takes all `refine_steps` -- are redone in FP32
+    and it goes on in FP32 to the FP32 stop; normals are FP32).
 
     The tracer keeps one device workspace per device; concurrent calls of one
     tracer object on different streams would share it and must not overlap."""
 
-    def __init__(self, args=None, form: str = None, refine_steps: int = 4, return_dist: float = 1e-2, **kw):
+    def __init__(self, args=None, form: str = None, refine_steps: int = 4, return_dist: float = 1e-2,
+                 near_dist: float = 1e-2, **kw):
         super().__init__(args, **kw)
         self.form = form or getattr(args, "form", None) or "fp32"
         self.refine_steps = refine_steps
         self.return_dist = return_dist
+        self.near_dist = near_dist
         self._ws = {}
         self.last_evals = None
         # When True, each call times its passes with HIP events (and waits
@@ -199,6 +202,7 @@ class SphereTracer(BaseTracer):
         p.normals = 1
         p.refine_steps = int(self.refine_steps)
         p.return_dist = float(self.return_dist)
+        p.near_dist = float(self.near_dist)
         pass_ms = (ctypes.c_float * 4)() if self.time_passes else None
         p.pass_ms = ctypes.cast(pass_ms, ctypes.c_void_p) if pass_ms is not None else None
         m = net.descriptor(form)

@@ -1,0 +1,81 @@
+"""The precision form on the benchmark model: the 5-LOD OctreeSDF fitted 200
+Adam steps to the seed-0 displaced icosphere, the 1280x720 pinhole frame at
+LOD 0..4 and uniform query points.  Hit-depth error, hit agreement and normal
+angle of both forms, and SDF error, against the FP64 oracle on a strided
+sample of the frame's rays.  Set NGLOD_REPORT to a file path to get the
+numbers as JSON."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from nglod_amd import SphereTracer, trainer, workloads
+from oracle import oracle
+from tests.helpers import arrays
+
+pytestmark = pytest.mark.gpu
+
+REPORT = {}
+
+
+@pytest.fixture(scope="module")
+def fitted(dev):
+    net, _ = trainer.fitted_icosphere_model(seed=0, device=dev, iterations=200)
+    yield net, arrays(net)
+    path = os.environ.get("NGLOD_REPORT")
+    if path:
+        with open(path, "w") as f:
+            json.dump(REPORT, f, indent=1)
+
+
+def _angle_deg(a, b):
+    c = (a * b).sum(1) / np.maximum(np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1), 1e-12)
+    return np.degrees(np.arccos(np.clip(c, -1.0, 1.0)))
+
+
+@pytest.mark.parametrize("lod", [0, 1, 2, 3, 4])
+def test_frame_depth_and_normals_vs_fp64(fitted, dev, lod):
+    net, m = fitted
+    net.lod = lod
+    o, d = workloads.camera_rays(1280, 720, device=dev)
+    idx = np.arange(0, o.shape[0], 53)  # 17 389 rays spread over the frame
+    ref = oracle.trace(m, o.cpu().numpy()[idx], d.cpu().numpy()[idx], lod, num_steps=128, prec="f64")
+    rep = {"sample_rays": int(idx.size), "fp64_hits": int(ref["hit"].sum())}
+    for form in ("fp32", "mixed"):
+        rb = SphereTracer(num_steps=128, form=form)(net, o, d)
+        torch.cuda.synchronize()
+        hit = rb.hit.cpu().numpy()[idx]
+        both = hit & ref["hit"]
+        derr = np.abs(rb.depth[:, 0].cpu().numpy()[idx][both] - ref["t"][both])
+        ang = _angle_deg(rb.normal.cpu().numpy()[idx][both], ref["normal"][both])
+        rep[form] = {"hit_agreement": float((hit == ref["hit"]).mean()), "depth_err_mean": float(derr.mean()),
+                     "depth_err_p99": float(np.percentile(derr, 99)), "normal_deg_median": float(np.median(ang)),
+                     "normal_deg_p99": float(np.percentile(ang, 99))}
+        assert (hit == ref["hit"]).mean() >= 0.99, (form, lod)
+        assert both.sum() > 1000
+        assert derr.mean() <= 1e-3, (form, lod, derr.mean())
+        assert np.median(ang) <= 0.5, (form, lod)
+    REPORT[f"frame_lod{lod}"] = rep
+
+
+def test_sdf_error_uniform_points(fitted, dev):
+    net, m = fitted
+    n = 1 << 22
+    g = torch.Generator().manual_seed(2)
+    x = (torch.rand(n, 3, generator=g) * 2.0 - 1.0)
+    xs = x[:: 64].numpy()  # 65 536 points for the FP64 oracle
+    rep = {"points": n, "sample": int(xs.shape[0])}
+    xd = x.to(dev)
+    for lod in range(5):
+        ref = oracle.sdf(m, xs, lod, "f64")
+        for form, code in (("fp32", 0), ("mixed", 1)):
+            got = net.sdf_hip(xd, lod, form=code)[::64, 0].cpu().numpy()
+            err = np.abs(got - ref)
+            rep[f"lod{lod}_{form}"] = {"max": float(err.max()), "mean": float(err.mean())}
+            if form == "fp32":
+                assert err.max() < 1e-5
+            else:
+                assert err.mean() < 3e-3
+    REPORT["sdf_uniform"] = rep
