@@ -71,9 +71,10 @@ typedef struct nglod_trace_params {
                              march stops on a surface test, its last
                              refine_steps evaluations are re-evaluated in FP32
                              and the march goes on in FP32 to its FP32 stop */
-    double return_dist;   /* NGLOD_MIXED only: an FP32 step that grows past this
-                             (the ray leaves the surface: a near miss) returns
-                             the ray to mixed precision, at most twice */
+    double return_dist;   /* NGLOD_MIXED only: if > 0, an FP32 step that grows
+                             past this (the ray leaves the surface: a near
+                             miss) returns the ray to mixed precision, at most
+                             twice; 0: the FP32 pass keeps every ray */
     double near_dist;     /* NGLOD_MIXED only: of the last refine_steps
                              evaluations, the stopping one and every earlier
                              one back to the first step of at least near_dist

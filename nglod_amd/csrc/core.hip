@@ -149,9 +149,11 @@ __global__ __launch_bounds__(256) void normals_fp32_kernel(ModelArgs m, int lod,
 }
 
 template <bool POSINV>
-__global__ __launch_bounds__(256) void trace_dense_fp32_kernel(TraceArgs a) {
+__global__ __launch_bounds__(kMarchBlock) void trace_dense_fp32_kernel(TraceArgs a) {
     __shared__ Fp32Lds lds;
+    __shared__ CoarseLds32 coarse;
     stage_fp32_lds(a.m, a.dec, &lds);
+    stage_coarse32(a.m, a.coarse, &coarse);
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(256) void trace_dense_fp32_kernel(TraceArgs a) {
         }
         if (__ballot(s.phase != PH_IDLE) == 0) break;
         const bool busy = s.phase != PH_IDLE;
-        const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz, a.coarse, &coarse);
         if (busy) {
             ++nevals;
             s.consume(a, d);
@@ -354,6 +356,7 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     a.return_dist = (float)params->return_dist;
     a.defer_normals = form == NGLOD_MIXED ? 1 : 0;
     a.replay = 0;
+    a.coarse = coarse_plan(m, lod);
     a.replay_steps = params->refine_steps < 1 ? 1 : params->refine_steps;
     a.near_dist = (float)params->near_dist;
     a.evals = (unsigned long long*)evals_out;
@@ -369,11 +372,11 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
         return rc ? rc : timer.finish();
     }
     if (m.pos_inv) {
-        const int grid = persistent_grid(trace_dense_fp32_kernel<true>, n, 256);
-        hipLaunchKernelGGL(trace_dense_fp32_kernel<true>, dim3(grid), dim3(256), 0, s, a);
+        const int grid = persistent_grid(trace_dense_fp32_kernel<true>, n, kMarchBlock);
+        hipLaunchKernelGGL(trace_dense_fp32_kernel<true>, dim3(grid), dim3(kMarchBlock), 0, s, a);
     } else {
-        const int grid = persistent_grid(trace_dense_fp32_kernel<false>, n, 256);
-        hipLaunchKernelGGL(trace_dense_fp32_kernel<false>, dim3(grid), dim3(256), 0, s, a);
+        const int grid = persistent_grid(trace_dense_fp32_kernel<false>, n, kMarchBlock);
+        hipLaunchKernelGGL(trace_dense_fp32_kernel<false>, dim3(grid), dim3(kMarchBlock), 0, s, a);
     }
     rc = check_launch("trace_dense_fp32_kernel");
     if (rc) return rc;

@@ -14,6 +14,7 @@
 // both tiles' B operands.
 #pragma once
 
+#include "coarse_lds.hpp"
 #include "device_common.hpp"
 
 namespace nglod {
@@ -101,10 +102,14 @@ __device__ __forceinline__ float layer2_tile(const f32x16_t& c, int h, uint32_t 
 // whatever the other lanes hold.
 template <bool POSINV>
 __device__ __forceinline__ float sdf_fp32_wave(const ModelArgs& m, int lod, const Fp32Lds* s, float b1,
-                                               uint32_t lane, bool active, float px, float py, float pz) {
+                                               uint32_t lane, bool active, float px, float py, float pz,
+                                               CoarsePlan cp = CoarsePlan{0, 0}, const CoarseLds32* cs = nullptr) {
     float f[kC];
     if (active) {
-        gather_fp32(m, lod, px, py, pz, f);
+        if (cp.levels > 0)
+            gather_fp32_coarse(m, lod, cp, cs, px, py, pz, f);
+        else
+            gather_fp32(m, lod, px, py, pz, f);
     } else {
 #pragma unroll
         for (int c = 0; c < kC; ++c) f[c] = 0.0f;

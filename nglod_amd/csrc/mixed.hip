@@ -13,7 +13,8 @@
 //             rewound steps and goes on in FP32 to its FP32 stop; a ray whose
 //             FP32 step grows past return_dist (the mixed stop was a near
 //             miss) is handed back and pass 1 resumes it.  Passes 1 and 2 run
-//             kMixedRounds times; the last pass 2 keeps every ray.
+//             kMixedRounds times (once when return_dist = 0); the last
+//             pass 2 keeps every ray.
 //     pass 3  FP32 normals of every finished hit, one hit per lane.
 #include "api_common.hpp"
 #include "fp32_wave.hpp"
@@ -21,6 +22,13 @@
 #include "trace_common.hpp"
 
 namespace nglod {
+
+// Passes 1 and 2 read every level from global memory: with LDS copies of the
+// coarse levels they measured slower (pass 1) or the same (pass 2) on the
+// 1280x720 LOD-4 frame.  Their lanes hold rays from all over the frame
+// (refills, hand-off order), so a wave's LDS reads spread over many corners.
+constexpr int kMixedBlock = 256;
+constexpr int kPass2Block = 256;
 
 // ---------------------------------------------------------------------------
 __global__ void pack_grid_kernel(const float* __restrict__ src, __half* __restrict__ dst, int64_t n) {
@@ -88,7 +96,7 @@ __device__ __forceinline__ void flush_lists(RayState& s, const TraceArgs& a, uin
 }
 
 template <bool POSINV>
-__global__ __launch_bounds__(256) void trace_mixed_pass1_kernel(TraceArgs a) {
+__global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArgs a) {
     __shared__ MixedLds lds;
     stage_mixed_lds(a.m, a.dec, &lds);
     __syncthreads();
@@ -210,7 +218,7 @@ __global__ __launch_bounds__(256) void trace_mixed_pass1_kernel(TraceArgs a) {
 }
 
 template <bool POSINV>
-__global__ __launch_bounds__(256) void trace_mixed_pass2_kernel(TraceArgs a) {
+__global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArgs a) {
     __shared__ Fp32Lds lds;
     stage_fp32_lds(a.m, a.dec, &lds);
     __syncthreads();
@@ -264,36 +272,43 @@ __global__ __launch_bounds__(256) void trace_mixed_pass2_kernel(TraceArgs a) {
 }
 
 // Pass 3: FP32 central-difference normals of the finished hits, one hit per
-// lane (six evaluations each, uniform work).
+// lane (six evaluations each, uniform work), persistent over the list.
 template <bool POSINV>
-__global__ __launch_bounds__(256) void normals_list_kernel(TraceArgs a) {
+__global__ __launch_bounds__(kMarchBlock) void normals_list_kernel(TraceArgs a) {
     const uint32_t total = *a.ncount;
     if ((int64_t)blockIdx.x * blockDim.x >= (int64_t)total) return;  // block-uniform, before the barrier
     __shared__ Fp32Lds lds;
+    __shared__ CoarseLds32 coarse;
     stage_fp32_lds(a.m, a.dec, &lds);
+    stage_coarse32(a.m, a.coarse, &coarse);
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < total;
-    if (__ballot(valid) == 0) return;  // whole wave past the end (no barrier follows)
     uint32_t nevals = 0;
-    NormalItem it = {0, 0.0f, 0.0f, 0.0f};
-    if (valid) it = a.nlist[i];
-    const float p[3] = {it.x0, it.x1, it.x2};
-    float g[3];
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < (int64_t)total; i0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = i0 + threadIdx.x;
+        const bool valid = i < total;
+        if (__ballot(valid) == 0) break;  // whole wave past the end (no barrier follows)
+        NormalItem it = {0, 0.0f, 0.0f, 0.0f};
+        if (valid) it = a.nlist[i];
+        const float p[3] = {it.x0, it.x1, it.x2};
+        float g[3];
 #pragma unroll 1
-    for (int ax = 0; ax < 3; ++ax) {
-        float pp[3] = {p[0] + 0.0f, p[1] + 0.0f, p[2] + 0.0f}, pm[3] = {p[0] - 0.0f, p[1] - 0.0f, p[2] - 0.0f};
-        pp[ax] = p[ax] + kFdH;
-        pm[ax] = p[ax] - kFdH;
-        const float fp = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pp[0], pp[1], pp[2]);
-        const float fm = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pm[0], pm[1], pm[2]);
-        g[ax] = (fp - fm) / kFdTwoH;
-    }
-    if (valid) {
-        nevals = 6;
-        normalize_grad(g[0], g[1], g[2], a.normal[3 * it.ray], a.normal[3 * it.ray + 1], a.normal[3 * it.ray + 2]);
+        for (int ax = 0; ax < 3; ++ax) {
+            float pp[3] = {p[0] + 0.0f, p[1] + 0.0f, p[2] + 0.0f}, pm[3] = {p[0] - 0.0f, p[1] - 0.0f, p[2] - 0.0f};
+            pp[ax] = p[ax] + kFdH;
+            pm[ax] = p[ax] - kFdH;
+            const float fp =
+                sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pp[0], pp[1], pp[2], a.coarse, &coarse);
+            const float fm =
+                sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pm[0], pm[1], pm[2], a.coarse, &coarse);
+            g[ax] = (fp - fm) / kFdTwoH;
+        }
+        if (valid) {
+            nevals += 6;
+            normalize_grad(g[0], g[1], g[2], a.normal[3 * it.ray], a.normal[3 * it.ray + 1],
+                           a.normal[3 * it.ray + 2]);
+        }
     }
     if (a.evals) {
         unsigned long long tot = nevals;
@@ -333,13 +348,17 @@ __global__ void reset_counters_kernel(uint32_t* c, int which) {
     }
 }
 
+// Rounds of (pass 1, pass 2) when near misses may go back to mixed precision
+// (return_dist > 0); with return_dist = 0 one round, pass 2 keeps every ray.
 constexpr int kMixedRounds = 3;
 
 template <bool POSINV>
 static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& timer) {
-    const int g1 = grid_for(trace_mixed_pass1_kernel<POSINV>, n, 256);
-    const int g2 = grid_for(trace_mixed_pass2_kernel<POSINV>, n, 256);
-    for (int r = 0; r < kMixedRounds; ++r) {
+    const int g1 = grid_for(trace_mixed_pass1_kernel<POSINV>, n, kMixedBlock);
+    const int g2 = grid_for(trace_mixed_pass2_kernel<POSINV>, n, kPass2Block);
+    const int g3 = grid_for(normals_list_kernel<POSINV>, n, kMarchBlock);
+    const int rounds = a.return_dist > 0.0f ? kMixedRounds : 1;
+    for (int r = 0; r < rounds; ++r) {
         a.from_returns = r > 0;
         if (r > 0) {
             hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, a.queue, 0);
@@ -348,7 +367,7 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
             timer.mark(kPassOther);
         }
         a.replay = 1;
-        hipLaunchKernelGGL(trace_mixed_pass1_kernel<POSINV>, dim3(g1), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(trace_mixed_pass1_kernel<POSINV>, dim3(g1), dim3(kMixedBlock), 0, s, a);
         a.replay = 0;
         int rc = check_launch("trace_mixed_pass1_kernel");
         if (rc) return rc;
@@ -357,13 +376,13 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
         rc = check_launch("reset_counters_kernel");
         if (rc) return rc;
         timer.mark(kPassOther);
-        a.allow_return = r + 1 < kMixedRounds;
-        hipLaunchKernelGGL(trace_mixed_pass2_kernel<POSINV>, dim3(g2), dim3(256), 0, s, a);
+        a.allow_return = r + 1 < rounds;
+        hipLaunchKernelGGL(trace_mixed_pass2_kernel<POSINV>, dim3(g2), dim3(kPass2Block), 0, s, a);
         rc = check_launch("trace_mixed_pass2_kernel");
         if (rc) return rc;
         timer.mark(kPassFp32);
     }
-    hipLaunchKernelGGL(normals_list_kernel<POSINV>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(normals_list_kernel<POSINV>, dim3(g3), dim3(kMarchBlock), 0, s, a);
     int rc = check_launch("normals_list_kernel");
     if (rc) return rc;
     timer.mark(kPassNormals);

@@ -2,6 +2,7 @@
 // persistent trace kernels.
 #pragma once
 
+#include "coarse_lds.hpp"
 #include "device_common.hpp"
 
 namespace nglod {
@@ -30,6 +31,9 @@ __device__ __forceinline__ void normalize_grad(float g0, float g1, float g2, flo
 // (workgroups are dispatched to XCDs round-robin, so blockIdx % 8 picks the
 // XCD-local range and its L2 sees one band of the image); a wave that drains
 // its own range moves on to the next ones.
+// Threads per block of the march kernels: one block of 12 waves per CU, all
+// sharing the decoder and coarse-level tables in LDS.
+constexpr int kMarchBlock = 768;
 constexpr int kQueues = 8;
 
 // PH_NQ: a finished hit whose normal is computed by a separate pass (the ray's
@@ -73,6 +77,7 @@ struct TraceArgs {
     int32_t* ray_steps;
     // Mixed form: rays handed from the mixed march to the FP32 pass.
     int replay;        // 1 in the mixed march: a surface stop becomes PH_REPLAY
+    CoarsePlan coarse; // leading levels the march kernels stage in LDS
     int replay_steps;  // at most this many evaluations rewound and re-evaluated in FP32
     float near_dist;   // ... back to the last step of at least this length (0: all replay_steps)
     struct Handoff* handoff;

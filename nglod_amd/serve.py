@@ -144,7 +144,7 @@ class TraceServer:
     priority); pool_bytes: the ray-pool capacity that caps admission."""
 
     def __init__(self, models, pool_bytes=1 << 30, slots=4, form="mixed", num_steps=128, min_dis=3e-4,
-                 far_clip=10.0, step_size=1.0, refine_steps=4, return_dist=1e-2, near_dist=1e-2,
+                 far_clip=10.0, step_size=1.0, refine_steps=4, return_dist=0.0, near_dist=1e-2,
                  keep_outputs=False):
         self.models = list(models)
         self.device = self.models[0].features[0].fm.device

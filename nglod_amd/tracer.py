@@ -151,7 +151,7 @@ class SphereTracer(BaseTracer):
     The tracer keeps one device workspace per device; concurrent calls of one
     tracer object on different streams would share it and must not overlap."""
 
-    def __init__(self, args=None, form: str = None, refine_steps: int = 4, return_dist: float = 1e-2,
+    def __init__(self, args=None, form: str = None, refine_steps: int = 4, return_dist: float = 0.0,
                  near_dist: float = 1e-2, **kw):
         super().__init__(args, **kw)
         self.form = form or getattr(args, "form", None) or "fp32"

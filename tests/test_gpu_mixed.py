@@ -58,8 +58,9 @@ def _angle_deg(a, b):
     return np.degrees(np.arccos(np.clip(c, -1.0, 1.0)))
 
 
-@pytest.mark.parametrize("lod,near_dist", [(2, 1e-2), (4, 1e-2), (4, 0.0)])
-def test_trace_mixed_vs_fp64(dev, lod, near_dist):
+@pytest.mark.parametrize("lod,near_dist,return_dist", [(2, 1e-2, 0.0), (4, 1e-2, 0.0), (4, 0.0, 0.0),
+                                                      (4, 1e-2, 1e-2)])
+def test_trace_mixed_vs_fp64(dev, lod, near_dist, return_dist):
     net = workloads.make_sphere_model(radius=0.6, num_lods=5, device="cpu", seed=4, noise=5e-3)
     m = arrays(net)
     o, d = workloads.camera_rays(160, 120, eye=(0.4, 0.8, 2.8), device="cpu")
@@ -67,7 +68,7 @@ def test_trace_mixed_vs_fp64(dev, lod, near_dist):
     ref = oracle.trace(m, o, d, lod, num_steps=128, prec="f64")
     netg = net.to(dev)
     netg.lod = lod
-    tr = SphereTracer(num_steps=128, form="mixed", near_dist=near_dist)
+    tr = SphereTracer(num_steps=128, form="mixed", near_dist=near_dist, return_dist=return_dist)
     rb = tr(netg, gpu(o, dev), gpu(d, dev))
     torch.cuda.synchronize()
     hit = rb.hit.cpu().numpy()
