@@ -1,17 +1,17 @@
-// The leading coarse levels of the feature grids, staged once per workgroup
-// in LDS, so that the gather of those levels costs LDS reads instead of
-// vector-memory (TA/TD) work.  At the standard resolutions LODs 0 and 1
-// (r = 4, 8) hold 125 + 729 = 854 corners: 109 KB in FP32, 55 KB in FP16.
+// The leading coarse levels of the FP32 feature grids, staged once per
+// workgroup in LDS, so that the gather of those levels costs LDS reads instead
+// of vector-memory (TA/TD) work.  At the standard resolutions LODs 0 and 1
+// (r = 4, 8) hold 125 + 729 = 854 corners: 109 KB.  Used by the FP32 march and
+// the normals pass, where it measured faster; the mixed march gathers every
+// level from global memory.
 //
-// Layout: corner `idx` (level 0 first, then level 1) is 8 (FP32) or 4 (FP16)
-// 16-byte chunks; logical chunk k is stored at chunk k ^ swizzle(idx), which
-// spreads the lanes of a ds_read_b128 that read different corners over all
-// 32 banks (a corner is 128 or 64 bytes, so without it every lane reading
-// chunk k of any corner would hit the same 4 banks).  Lanes reading the same
-// corner read the same address and are served by broadcast.
+// Layout: corner `idx` (level 0 first, then level 1) is 8 16-byte chunks;
+// logical chunk k is stored at chunk k ^ (idx & 7), which spreads the lanes of
+// a ds_read_b128 that read different corners over all 32 banks (a corner is
+// 128 bytes, so without it every lane reading chunk k of any corner would hit
+// the same 4 banks).  Lanes reading the same corner read the same address and
+// are served by broadcast.
 #pragma once
-
-#include <hip/hip_fp16.h>
 
 #include "device_common.hpp"
 
@@ -22,12 +22,8 @@ constexpr int kCoarseCorners = 854;
 struct CoarseLds32 {
     uint4 c[kCoarseCorners * 8];
 };
-struct CoarseLds16 {
-    uint4 c[kCoarseCorners * 4];
-};
 
 __device__ __forceinline__ int swz32(int idx, int k) { return idx * 8 + (k ^ (idx & 7)); }
-__device__ __forceinline__ int swz16(int idx, int k) { return idx * 4 + (k ^ ((idx >> 1) & 3)); }
 
 // Number of leading levels (at most 2, none beyond lod) whose corners fit the
 // LDS budget, and the first corner of each.
@@ -66,23 +62,6 @@ __device__ __forceinline__ void stage_coarse32(const ModelArgs& m, CoarsePlan p,
         const int l = idx < n0 ? 0 : 1;
         const int local = l == 0 ? idx : idx - p.off1;
         s->c[swz32(idx, k)] = reinterpret_cast<const uint4*>(m.grid[l] + (size_t)local * kC)[k];
-    }
-}
-
-__device__ __forceinline__ void stage_coarse16(const ModelArgs& m, CoarsePlan p, CoarseLds16* s) {
-    if (p.levels == 0) return;
-    const int s0 = m.res[0] + 1;
-    const int n0 = s0 * s0 * s0;
-    int total = n0;
-    if (p.levels > 1) {
-        const int s1 = m.res[1] + 1;
-        total += s1 * s1 * s1;
-    }
-    for (int j = threadIdx.x; j < total * 4; j += blockDim.x) {
-        const int idx = j >> 2, k = j & 3;
-        const int l = idx < n0 ? 0 : 1;
-        const int local = l == 0 ? idx : idx - p.off1;
-        s->c[swz16(idx, k)] = reinterpret_cast<const uint4*>(m.grid_h[l] + (size_t)local * kC)[k];
     }
 }
 

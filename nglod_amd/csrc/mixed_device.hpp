@@ -15,7 +15,6 @@
 
 #include <hip/hip_fp16.h>
 
-#include "coarse_lds.hpp"
 #include "device_common.hpp"
 
 namespace nglod {
@@ -93,58 +92,6 @@ __device__ __forceinline__ void gather_fp16(const ModelArgs& m, int lod, float p
     }
 }
 
-__device__ __forceinline__ void corner_hfma_lds(__half2 (&acc)[kC / 2], const CoarseLds16* s, int idx, __half2 w) {
-#pragma unroll
-    for (int k = 0; k < kC / 8; ++k) {
-        const uint4 g = s->c[swz16(idx, k)];
-        acc[4 * k + 0] = __hfma2(__builtin_bit_cast(__half2, g.x), w, acc[4 * k + 0]);
-        acc[4 * k + 1] = __hfma2(__builtin_bit_cast(__half2, g.y), w, acc[4 * k + 1]);
-        acc[4 * k + 2] = __hfma2(__builtin_bit_cast(__half2, g.z), w, acc[4 * k + 2]);
-        acc[4 * k + 3] = __hfma2(__builtin_bit_cast(__half2, g.w), w, acc[4 * k + 3]);
-    }
-}
-
-// gather_fp16 with levels < p.levels read from LDS (same corners, weights and
-// FMA order: bit-identical).
-__device__ __forceinline__ void gather_fp16_coarse(const ModelArgs& m, int lod, CoarsePlan p, const CoarseLds16* s,
-                                                   float px, float py, float pz, __half2 (&acc)[kC / 2]) {
-#pragma unroll
-    for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
-    for (int l = 0; l <= lod; ++l) {
-        const int r = m.res[l];
-        const int S = r + 1;
-        const AxisW ax = axis_weights(px, r);
-        const AxisW ay = axis_weights(py, r);
-        const AxisW az = axis_weights(pz, r);
-        const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
-        const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
-        if (l < p.levels) {
-            const int i0 = (l == 0 ? 0 : p.off1) + (az.i0 * S + ay.i0) * S + ax.i0;
-            const int sy = S, sz = S * S;
-            corner_hfma_lds(acc, s, i0, __float2half2_rn(wxy00 * az.w0));
-            corner_hfma_lds(acc, s, i0 + 1, __float2half2_rn(wxy10 * az.w0));
-            corner_hfma_lds(acc, s, i0 + sy, __float2half2_rn(wxy01 * az.w0));
-            corner_hfma_lds(acc, s, i0 + sy + 1, __float2half2_rn(wxy11 * az.w0));
-            corner_hfma_lds(acc, s, i0 + sz, __float2half2_rn(wxy00 * az.w1));
-            corner_hfma_lds(acc, s, i0 + sz + 1, __float2half2_rn(wxy10 * az.w1));
-            corner_hfma_lds(acc, s, i0 + sz + sy, __float2half2_rn(wxy01 * az.w1));
-            corner_hfma_lds(acc, s, i0 + sz + sy + 1, __float2half2_rn(wxy11 * az.w1));
-        } else {
-            const __half* base = reinterpret_cast<const __half*>(m.grid_h[l]) +
-                                 ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
-            const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
-            corner_hfma(acc, base, __float2half2_rn(wxy00 * az.w0));
-            corner_hfma(acc, base + sx, __float2half2_rn(wxy10 * az.w0));
-            corner_hfma(acc, base + sy, __float2half2_rn(wxy01 * az.w0));
-            corner_hfma(acc, base + sy + sx, __float2half2_rn(wxy11 * az.w0));
-            corner_hfma(acc, base + sz, __float2half2_rn(wxy00 * az.w1));
-            corner_hfma(acc, base + sz + sx, __float2half2_rn(wxy10 * az.w1));
-            corner_hfma(acc, base + sz + sy, __float2half2_rn(wxy01 * az.w1));
-            corner_hfma(acc, base + sz + sy + sx, __float2half2_rn(wxy11 * az.w1));
-        }
-    }
-}
-
 __device__ __forceinline__ uint32_t pack_bf16(__half2 h) {
     const float2 f = __half22float2(h);
     const bf16x2v v = {(__bf16)f.x, (__bf16)f.y};
@@ -187,14 +134,10 @@ __device__ __forceinline__ float layer2_partial(const f32x16& c, int h, uint32_t
 // depends only on its own query.
 template <bool POSINV>
 __device__ __forceinline__ float sdf_mixed_wave(const ModelArgs& m, int lod, const MixedLds* s, float b1,
-                                                uint32_t lane, bool active, float px, float py, float pz,
-                                                CoarsePlan cp = CoarsePlan{0, 0}, const CoarseLds16* cs = nullptr) {
+                                                uint32_t lane, bool active, float px, float py, float pz) {
     __half2 acc[kC / 2];
     if (active) {
-        if (cp.levels > 0)
-            gather_fp16_coarse(m, lod, cp, cs, px, py, pz, acc);
-        else
-            gather_fp16(m, lod, px, py, pz, acc);
+        gather_fp16(m, lod, px, py, pz, acc);
     } else {
 #pragma unroll
         for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
