@@ -199,8 +199,15 @@ struct RayState {
         }
     }
 
-    // Pass 2 of the mixed form: continue a ray from its hand-off record.
+    // Pass 2 of the mixed form: continue a ray from its hand-off record.  A
+    // record whose ray index is out of range (never the case for a written
+    // record) leaves the lane idle instead of indexing outside the arrays.
     __device__ __forceinline__ void resume(const TraceArgs& a, const Handoff& h) {
+        if (h.ray < 0 || (int64_t)h.ray >= a.qend[kQueues - 1]) {
+            phase = PH_IDLE;
+            ray = -1;
+            return;
+        }
         ray = h.ray;
         ox = a.ray_o[3 * ray];
         oy = a.ray_o[3 * ray + 1];
@@ -274,6 +281,9 @@ struct RayState {
             ++it;
             ++steps;
             check(a);
+        } else if (phase < PH_N0 || phase > PH_N5) {
+            // not an evaluating phase: end the ray rather than loop on it
+            finish(a);
         } else {
             const int k = phase - PH_N0;  // 0..5: axis k/2, + tap then - tap
             const int axis = k >> 1;
