@@ -237,7 +237,7 @@ def main():
             passes = {
                 "trace_mixed_pass1_kernel": (pass_ms[0], ev[1] * b16 + n * 24),
                 "trace_mixed_pass2_kernel": (pass_ms[1], n_fp32 * b32),
-                "normals_list_kernel": (pass_ms[2], 6 * hits * b32 + hits * 24),
+                "normals_pass_kernel": (pass_ms[2], 6 * hits * b32 + hits * 13),
             }
         return {
             "rays_per_s": n * args.steps / wall,

@@ -157,34 +157,14 @@ __global__ __launch_bounds__(kMarchBlock) void trace_dense_fp32_kernel(TraceArgs
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
-    int q = blockIdx.x % kQueues;
-    int tried = 0;
-    bool drained = false;
+    RayFeed feed;
+    feed.init();
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
     uint32_t nevals = 0;
     for (;;) {
-        // Refill idle lanes until every lane is busy or all queues are drained.
-        for (;;) {
-            const bool need = (s.phase == PH_IDLE) && !drained;
-            const uint64_t mask = __ballot(need);
-            if (mask == 0) break;
-            const int cnt = __popcll(mask);
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(&a.queue[q], (uint32_t)cnt);
-            base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-            const int64_t qsize = a.qend[q] - a.qbeg[q];
-            if (need) {
-                const int64_t idx = (int64_t)base + (int64_t)mask_rank(mask);
-                if (idx < qsize) s.start(a, a.qbeg[q] + idx);
-            }
-            if ((int64_t)base + cnt >= qsize) {
-                q = (q + 1) % kQueues;
-                if (++tried == kQueues) drained = true;
-            }
-        }
+        feed.refill<0>(s, a, lane);
         if (__ballot(s.phase != PH_IDLE) == 0) break;
         const bool busy = s.phase != PH_IDLE;
         const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz, a.coarse, &coarse);
@@ -300,11 +280,19 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
     return check_launch("normals_fp32_kernel");
 }
 
-static constexpr size_t kWsCounters = 256;  // queue[8], handoff_count/queue, ncount, ret_count/queue
+// Workspace: 256 bytes of counters (the ray queues, then the claim counters
+// of pass 2, of a pass 1 over returned rays and of the normals pass, kQueues
+// each); for the
+// mixed form then hand-off records [n], returned records [n], hand-off flags
+// [n] and return flags [n] (each flag array padded to 256 bytes).
+static constexpr size_t kWsCounters = 256;
+static constexpr int kCounterWords = 4 * kQueues;
+
+static size_t flag_bytes(int64_t n) { return ((size_t)n + 255) / 256 * 256; }
 
 size_t nglod_trace_dense_workspace_size(int64_t n, int32_t form) {
     if (n < 0) n = 0;
-    return kWsCounters + (form == NGLOD_MIXED ? (size_t)n * (2 * sizeof(Handoff) + sizeof(NormalItem)) : 0);
+    return kWsCounters + (form == NGLOD_MIXED ? (size_t)n * 2 * sizeof(Handoff) + 2 * flag_bytes(n) : 0);
 }
 
 int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const float* ray_d, int64_t n, int32_t lod,
@@ -343,19 +331,22 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     a.hit = hit;
     a.normal = normal;
     a.queue = (uint32_t*)workspace;
-    a.handoff_count = a.queue + kQueues;
-    a.handoff_queue = a.queue + kQueues + 1;
-    a.ncount = a.queue + kQueues + 2;
-    a.handoff = (Handoff*)((char*)workspace + kWsCounters);
-    a.ret_count = a.queue + kQueues + 3;
-    a.ret_queue = a.queue + kQueues + 4;
-    a.ret = (Handoff*)((char*)workspace + kWsCounters + (size_t)n * sizeof(Handoff));
-    a.nlist = (NormalItem*)((char*)workspace + kWsCounters + 2 * (size_t)n * sizeof(Handoff));
-    a.from_returns = 0;
-    a.allow_return = 0;
+    a.handoff_queue = a.queue + kQueues;
+    a.ret_queue = a.queue + 2 * kQueues;
+    a.normal_queue = a.queue + 3 * kQueues;
+    a.handoff = nullptr;
+    a.ret = nullptr;
+    a.hflag = nullptr;
+    a.rflag = nullptr;
+    if (form == NGLOD_MIXED) {
+        char* w = (char*)workspace + kWsCounters;
+        a.handoff = (Handoff*)w;
+        a.ret = (Handoff*)(w + (size_t)n * sizeof(Handoff));
+        a.hflag = (uint8_t*)(w + 2 * (size_t)n * sizeof(Handoff));
+        a.rflag = a.hflag + flag_bytes(n);
+    }
     a.return_dist = (float)params->return_dist;
     a.defer_normals = form == NGLOD_MIXED ? 1 : 0;
-    a.replay = 0;
     a.coarse = coarse_plan(m, lod);
     a.replay_steps = params->refine_steps < 1 ? 1 : params->refine_steps;
     a.near_dist = (float)params->near_dist;
@@ -363,7 +354,7 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     a.ray_steps = ray_steps;
     split_queues(n, a.qbeg, a.qend);
     PassTimer timer(params->pass_ms, kPassSlots, s);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, a.queue, kQueues + 5, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, a.queue, kCounterWords, 0u);
     rc = check_launch("fill_u32_kernel");
     if (rc) return rc;
     timer.mark(kPassOther);

@@ -6,16 +6,20 @@
 //     pass 1  persistent mixed march.  When a ray stops on a surface test
 //             (|d| <= min_dis or the oscillation test) it rewinds its last
 //             near-surface evaluations (at most replay_steps, back to the
-//             last step of at least near_dist) and is appended to a hand-off list
-//             (ballot + one atomic per wave); finished hits go to the
-//             normal list.
-//     pass 2  persistent FP32 kernel over the hand-off list: re-evaluates the
-//             rewound steps and goes on in FP32 to its FP32 stop; a ray whose
-//             FP32 step grows past return_dist (the mixed stop was a near
-//             miss) is handed back and pass 1 resumes it.  Passes 1 and 2 run
-//             kMixedRounds times (once when return_dist = 0); the last
-//             pass 2 keeps every ray.
-//     pass 3  FP32 normals of every finished hit, one hit per lane.
+//             last step of at least near_dist) and writes its hand-off
+//             record at handoff[ray] (flag hflag[ray]); finished hits are
+//             written out without their normal.
+//     pass 2  persistent FP32 kernel over the hand-off records, claimed in
+//             ray order: re-evaluates the rewound steps and goes on in FP32
+//             to its FP32 stop; a ray whose FP32 step grows past return_dist
+//             (the mixed stop was a near miss) is handed back (ret[ray],
+//             rflag[ray]) and pass 1 resumes it.  Passes 1 and 2 run
+//             kMixedRounds times (once when return_dist = 0); the last pass 2
+//             keeps every ray.
+//     pass 3  FP32 normals of every hit, one hit per lane.
+// No pass appends to a shared list: the records are indexed by ray, so the
+// only atomics are the claim counters (one per wave per refill).  A single
+// append counter hit by every wave serialised the march kernels measurably.
 #include "api_common.hpp"
 #include "fp32_wave.hpp"
 #include "mixed_device.hpp"
@@ -27,6 +31,17 @@ namespace nglod {
 // coarse levels (FP16 or FP32 layout) they measured slower (pass 1, by 10-20%)
 // or the same (pass 2) on the 1280x720 LOD-4 frame.
 constexpr int kMixedBlock = 256;
+// Pass 1 claims rays 64 ids at a time (the FP32 march claims per idle lane):
+// measured faster for the mixed march, slightly slower for the FP32 one.
+#ifndef NGLOD_PASS1_CHUNK
+#define NGLOD_PASS1_CHUNK 64
+#endif
+constexpr int kPass1Chunk = NGLOD_PASS1_CHUNK;
+// Passes over ray-indexed records claim 64 ids (about 40 records) per atomic.
+#ifndef NGLOD_CLAIM_CHUNK
+#define NGLOD_CLAIM_CHUNK 64
+#endif
+constexpr int kClaimChunk = NGLOD_CLAIM_CHUNK;
 constexpr int kPass2Block = 256;
 
 // ---------------------------------------------------------------------------
@@ -68,214 +83,162 @@ __global__ __launch_bounds__(256) void sdf_mixed_kernel(ModelArgs m, int lod, in
     if (valid) out[i] = d;
 }
 
-// ---------------------------------------------------------------------------
-// Wave-aggregated append: lanes with `want` get consecutive slots of a list
-// whose fill counter is `count` (one atomic per wave).
-__device__ __forceinline__ uint32_t append_slot(uint32_t* count, uint32_t lane, bool want, bool& any) {
-    const uint64_t mask = __ballot(want);
-    any = mask != 0;
-    if (!any) return 0;
-    const int cnt = __popcll(mask);
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(count, (uint32_t)cnt);
-    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-    return base + mask_rank(mask);
+// Pre-evaluation (t, D) of a ray's last four mixed-precision evaluations (a
+// shift register), and the rewind at a mixed surface stop.
+struct MarchHistory {
+    float t0, t1, t2, t3, d0, d1, d2, d3;
+    int n;      // evaluations recorded since the ray started (or came back)
+    bool init;  // the first recorded evaluation was the box-entry one
+
+    __device__ __forceinline__ void clear() {
+        n = 0;
+        init = false;
+    }
+
+    __device__ __forceinline__ void push(const RayState& s) {
+        if (s.phase == PH_INIT) {
+            n = 0;
+            init = true;
+        }
+        t3 = t2; t2 = t1; t1 = t0; t0 = s.t;
+        d3 = d2; d2 = d1; d1 = d0; d0 = s.D;
+        ++n;
+    }
+
+    // Rewind j evaluations (at most replay_steps, back to the box-entry
+    // evaluation if the ray made fewer steps): the stopping one and those
+    // before it back to (not including) the last step of at least near_dist.
+    // A step that long was taken away from the surface, where the
+    // mixed-precision error cannot carry the ray past it.  Leaves the ray with
+    // a pending evaluation (phase INIT or MARCH) for the FP32 form.
+    __device__ __forceinline__ void rewind(RayState& s, const TraceArgs& a) const {
+        const int jmax = n < a.replay_steps ? n : a.replay_steps;
+        int j = 1;
+        if (a.near_dist <= 0.0f) {
+            j = jmax;
+        } else {
+            if (j < jmax && fabsf(d0) < a.near_dist) ++j;
+            if (j == 2 && j < jmax && fabsf(d1) < a.near_dist) ++j;
+            if (j == 3 && j < jmax && fabsf(d2) < a.near_dist) ++j;
+        }
+        const float tj = j == 1 ? t0 : j == 2 ? t1 : j == 3 ? t2 : t3;
+        const float dj = j == 1 ? d0 : j == 2 ? d1 : j == 3 ? d2 : d3;
+        if (init && j == n) {
+            const BoxEntry e = box_entry(s.ox, s.oy, s.oz, s.dx, s.dy, s.dz);
+            s.phase = PH_INIT;
+            s.x0 = e.x;
+            s.x1 = e.y;
+            s.x2 = e.z;
+            s.t = e.t;
+            s.it = 0;
+            s.steps = 0;
+        } else {
+            s.phase = PH_MARCH;
+            s.t = tj;
+            s.D = dj;
+            s.it -= j;
+            s.steps -= j;
+            s.x0 = s.ox + s.dx * tj;
+            s.x1 = s.oy + s.dy * tj;
+            s.x2 = s.oz + s.dz * tj;
+        }
+        s.qx = s.x0;
+        s.qy = s.x1;
+        s.qz = s.x2;
+    }
+};
+
+__device__ __forceinline__ void add_evals(const TraceArgs& a, int slot, uint32_t nevals, uint32_t lane) {
+    if (!a.evals) return;
+    unsigned long long tot = nevals;
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (lane == 0 && tot) atomicAdd(a.evals + slot, tot);
 }
 
-// Moves lanes that need FP32 work (`refine`) to the hand-off list and
-// finished hits (phase PH_NQ) to the normal list.
-__device__ __forceinline__ void flush_lists(RayState& s, const TraceArgs& a, uint32_t lane, bool refine) {
-    bool any;
-    const uint32_t hs = append_slot(a.handoff_count, lane, refine, any);
-    if (refine) s.hand_off(a, hs);
-    const bool nq = s.phase == PH_NQ;
-    const uint32_t ns = append_slot(a.ncount, lane, nq, any);
-    if (nq) s.queue_normal(a, ns);
-}
-
-template <bool POSINV>
+// RESUME: a later round, over the rays pass 2 handed back (ret, rflag)
+// instead of the frame's rays.  A separate instantiation: compiled into the
+// first-round kernel, the resume path slowed its march by half even unused.
+template <bool POSINV, bool RESUME>
 __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArgs a) {
     __shared__ MixedLds lds;
     stage_mixed_lds(a.m, a.dec, &lds);
     __syncthreads();
     const uint32_t lane = lane_id();
     const float b1 = a.m.b1[a.dec][0];
-    int q = blockIdx.x % kQueues;
-    int tried = 0;
-    bool drained = false;
+    RayFeed feed;
+    feed.init();
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
     uint32_t nevals = 0;
-    float ht0 = 0, ht1 = 0, ht2 = 0, ht3 = 0, hd0 = 0, hd1 = 0, hd2 = 0, hd3 = 0;
-    int hist_n = 0;
-    bool hist_init = false;
-    bool replay = false;
-    const uint32_t ret_total = a.from_returns ? *a.ret_count : 0u;
+    MarchHistory hist;
+    hist.clear();
     for (;;) {
-        for (;;) {
-            const bool need = (s.phase == PH_IDLE) && !drained;
-            const uint64_t mask = __ballot(need);
-            if (mask == 0) break;
-            const int cnt = __popcll(mask);
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t base = 0;
-            if (a.from_returns) {
-                if ((int)lane == leader) base = atomicAdd(a.ret_queue, (uint32_t)cnt);
-                base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-                if (need) {
-                    const uint32_t idx = base + mask_rank(mask);
-                    if (idx < ret_total) {
-                        s.resume(a, a.ret[idx]);
-                        hist_n = 0;
-                        hist_init = false;
-                    }
-                }
-                if (base + (uint32_t)cnt >= ret_total) drained = true;
-            } else {
-                if ((int)lane == leader) base = atomicAdd(&a.queue[q], (uint32_t)cnt);
-                base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-                const int64_t qsize = a.qend[q] - a.qbeg[q];
-                if (need) {
-                    const int64_t idx = (int64_t)base + (int64_t)mask_rank(mask);
-                    if (idx < qsize) s.start(a, a.qbeg[q] + idx);
-                }
-                if ((int64_t)base + cnt >= qsize) {
-                    q = (q + 1) % kQueues;
-                    if (++tried == kQueues) drained = true;
-                }
+        if (RESUME) {
+            const int64_t id = feed.next<kClaimChunk>(a.ret_queue, a, a.rflag, lane, s.phase == PH_IDLE);
+            if (id >= 0) {
+                s.resume(a, a.ret[id]);
+                hist.clear();
             }
-            // origins inside the box finish at once as hits
-            flush_lists(s, a, lane, false);
+        } else {
+            feed.refill<kPass1Chunk>(s, a, lane);
         }
         if (__ballot(s.phase != PH_IDLE) == 0) break;
         const bool busy = s.phase != PH_IDLE;
         const float d = sdf_mixed_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
-            // pre-evaluation state of the last kReplay evaluations (shift register)
-            if (s.phase == PH_INIT) {
-                hist_n = 0;
-                hist_init = true;
-            }
-            ht3 = ht2; ht2 = ht1; ht1 = ht0; ht0 = s.t;
-            hd3 = hd2; hd2 = hd1; hd1 = hd0; hd0 = s.D;
-            ++hist_n;
-            s.consume(a, d);
+            hist.push(s);
+            s.consume(a, d, true);
             if (s.phase == PH_REPLAY) {
-                // Rewind j evaluations (at most replay_steps, back to the
-                // box-entry evaluation if the ray made fewer steps): the
-                // stopping one and those before it back to (not including)
-                // the last step of at least near_dist.  A step that long was
-                // taken away from the surface, where the mixed-precision
-                // error cannot carry the ray past it.
-                const int jmax = hist_n < a.replay_steps ? hist_n : a.replay_steps;
-                int j = 1;
-                if (a.near_dist <= 0.0f) {
-                    j = jmax;
-                } else {
-                    if (j < jmax && fabsf(hd0) < a.near_dist) ++j;
-                    if (j == 2 && j < jmax && fabsf(hd1) < a.near_dist) ++j;
-                    if (j == 3 && j < jmax && fabsf(hd2) < a.near_dist) ++j;
-                }
-                const float tj = j == 1 ? ht0 : j == 2 ? ht1 : j == 3 ? ht2 : ht3;
-                const float dj = j == 1 ? hd0 : j == 2 ? hd1 : j == 3 ? hd2 : hd3;
-                if (hist_init && j == hist_n) {
-                    const BoxEntry e = box_entry(s.ox, s.oy, s.oz, s.dx, s.dy, s.dz);
-                    s.phase = PH_INIT;
-                    s.x0 = e.x;
-                    s.x1 = e.y;
-                    s.x2 = e.z;
-                    s.t = e.t;
-                    s.it = 0;
-                    s.steps = 0;
-                } else {
-                    s.phase = PH_MARCH;
-                    s.t = tj;
-                    s.D = dj;
-                    s.it -= j;
-                    s.steps -= j;
-                    s.x0 = s.ox + s.dx * tj;
-                    s.x1 = s.oy + s.dy * tj;
-                    s.x2 = s.oz + s.dz * tj;
-                }
-                s.qx = s.x0;
-                s.qy = s.x1;
-                s.qz = s.x2;
-                replay = true;
+                hist.rewind(s, a);
+                s.hand_off_to(a.handoff, a.hflag);
             }
         }
-        flush_lists(s, a, lane, replay);
-        replay = false;
     }
-    if (a.evals) {
-        unsigned long long tot = nevals;
-        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-        if (lane == 0) atomicAdd(a.evals + 1, tot);
-    }
+    add_evals(a, 1, nevals, lane);
 }
 
-template <bool POSINV>
+// RETURNS: near misses may be handed back to a later pass 1 (not in the last
+// round); a separate instantiation, as for pass 1.
+template <bool POSINV, bool RETURNS>
 __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArgs a) {
     __shared__ Fp32Lds lds;
     stage_fp32_lds(a.m, a.dec, &lds);
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
-    const uint32_t total = *a.handoff_count;
-    bool drained = false;
+    RayFeed feed;
+    feed.init();
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
     uint32_t nevals = 0;
     for (;;) {
-        {
-            const bool need = (s.phase == PH_IDLE) && !drained;
-            const uint64_t mask = __ballot(need);
-            if (mask != 0) {
-                const int cnt = __popcll(mask);
-                const int leader = __ffsll((unsigned long long)mask) - 1;
-                uint32_t base = 0;
-                if ((int)lane == leader) base = atomicAdd(a.handoff_queue, (uint32_t)cnt);
-                base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-                if (need) {
-                    const uint32_t idx = base + mask_rank(mask);
-                    if (idx < total) s.resume(a, a.handoff[idx]);
-                }
-                if (base + (uint32_t)cnt >= total) drained = true;
-            }
-        }
+        const int64_t id = feed.next<kClaimChunk>(a.handoff_queue, a, a.hflag, lane, s.phase == PH_IDLE);
+        if (id >= 0) s.resume(a, a.handoff[id]);
+#ifdef NGLOD_DIAG_PASS2
+        if (id >= 0) s.steps = 1000;
+#endif
         if (__ballot(s.phase != PH_IDLE) == 0) break;
         const bool busy = s.phase != PH_IDLE;
         const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
             s.consume(a, d);
+            // hand back only a ray that moves away from the surface (step growing)
+            if (RETURNS && s.phase == PH_MARCH && fabsf(s.D) > a.return_dist && fabsf(s.D) > fabsf(s.Dp))
+                s.hand_off_to(a.ret, a.rflag);
         }
-        bool any;
-        // hand back only a ray that moves away from the surface (step growing)
-        const bool back = a.allow_return && s.phase == PH_MARCH && fabsf(s.D) > a.return_dist &&
-                          fabsf(s.D) > fabsf(s.Dp);
-        const uint32_t rs = append_slot(a.ret_count, lane, back, any);
-        if (back) s.hand_off_to(a.ret, rs);
-        const bool nq = s.phase == PH_NQ;
-        const uint32_t ns = append_slot(a.ncount, lane, nq, any);
-        if (nq) s.queue_normal(a, ns);
     }
-    if (a.evals) {
-        unsigned long long tot = nevals;
-        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-        if (lane == 0) atomicAdd(a.evals, tot);
-    }
+    add_evals(a, 0, nevals, lane);
 }
 
-// Pass 3: FP32 central-difference normals of the finished hits, one hit per
-// lane (six evaluations each, uniform work), persistent over the list.
+// Pass 3: FP32 central-difference normals of the hits (hit[ray] = 1, the hit
+// point in x[ray]), one hit per lane, six evaluations each; waves claim
+// consecutive rays, so their hits are neighbours in the frame.
 template <bool POSINV>
-__global__ __launch_bounds__(kMarchBlock) void normals_list_kernel(TraceArgs a) {
-    const uint32_t total = *a.ncount;
-    if ((int64_t)blockIdx.x * blockDim.x >= (int64_t)total) return;  // block-uniform, before the barrier
+__global__ __launch_bounds__(kMarchBlock) void normals_pass_kernel(TraceArgs a) {
     __shared__ Fp32Lds lds;
     __shared__ CoarseLds32 coarse;
     stage_fp32_lds(a.m, a.dec, &lds);
@@ -283,14 +246,20 @@ __global__ __launch_bounds__(kMarchBlock) void normals_list_kernel(TraceArgs a) 
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
+    RayFeed feed;
+    feed.init();
     uint32_t nevals = 0;
-    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < (int64_t)total; i0 += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = i0 + threadIdx.x;
-        const bool valid = i < total;
-        if (__ballot(valid) == 0) break;  // whole wave past the end (no barrier follows)
-        NormalItem it = {0, 0.0f, 0.0f, 0.0f};
-        if (valid) it = a.nlist[i];
-        const float p[3] = {it.x0, it.x1, it.x2};
+    for (;;) {
+        // uniform work per hit: one range in frame order, a claim per idle lane
+        const int64_t id = feed.next<0, false>(a.normal_queue, a, a.hit, lane, true);
+        const bool valid = id >= 0;
+        if (__ballot(valid) == 0) break;
+        float p[3] = {0.0f, 0.0f, 0.0f};
+        if (valid) {
+            p[0] = a.x[3 * id];
+            p[1] = a.x[3 * id + 1];
+            p[2] = a.x[3 * id + 2];
+        }
         float g[3];
 #pragma unroll 1
         for (int ax = 0; ax < 3; ++ax) {
@@ -305,15 +274,10 @@ __global__ __launch_bounds__(kMarchBlock) void normals_list_kernel(TraceArgs a) 
         }
         if (valid) {
             nevals += 6;
-            normalize_grad(g[0], g[1], g[2], a.normal[3 * it.ray], a.normal[3 * it.ray + 1],
-                           a.normal[3 * it.ray + 2]);
+            normalize_grad(g[0], g[1], g[2], a.normal[3 * id], a.normal[3 * id + 1], a.normal[3 * id + 2]);
         }
     }
-    if (a.evals) {
-        unsigned long long tot = nevals;
-        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-        if ((threadIdx.x & 63) == 0 && tot) atomicAdd(a.evals, tot);
-    }
+    add_evals(a, 0, nevals, lane);
 }
 
 template <typename K>
@@ -334,17 +298,16 @@ int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64
     return check_launch("sdf_mixed_kernel");
 }
 
-// Counter slots (uint32 in the workspace): 0..7 ray queues, 8 handoff_count,
-// 9 handoff_queue, 10 ncount, 11 ret_count, 12 ret_queue.
-__global__ void reset_counters_kernel(uint32_t* c, int which) {
-    if (threadIdx.x != 0) return;
-    if (which == 0) {  // before a pass 1 that reads the return list
-        c[8] = 0;
-        c[9] = 0;
-        c[12] = 0;
-    } else {  // before a pass 2
-        c[11] = 0;
-    }
+__global__ void clear_words_kernel(uint32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// Clears `words` uint32 of flags and the kQueues claim counters of a pass.
+__global__ void clear_round_kernel(uint32_t* flags, int words, uint32_t* counters) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) flags[i] = 0u;
+    if (i < kQueues) counters[i] = 0u;
 }
 
 // Rounds of (pass 1, pass 2) when near misses may go back to mixed precision
@@ -353,38 +316,46 @@ constexpr int kMixedRounds = 3;
 
 template <bool POSINV>
 static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& timer) {
-    const int g1 = grid_for(trace_mixed_pass1_kernel<POSINV>, n, kMixedBlock);
-    const int g2 = grid_for(trace_mixed_pass2_kernel<POSINV>, n, kPass2Block);
-    const int g3 = grid_for(normals_list_kernel<POSINV>, n, kMarchBlock);
+    const int g1 = grid_for(trace_mixed_pass1_kernel<POSINV, false>, n, kMixedBlock);
+    const int g1r = grid_for(trace_mixed_pass1_kernel<POSINV, true>, n, kMixedBlock);
+    const int g2 = grid_for(trace_mixed_pass2_kernel<POSINV, false>, n, kPass2Block);
+    const int g3 = grid_for(normals_pass_kernel<POSINV>, n, kMarchBlock);
+    const int words = (int)((n + 3) / 4);
+    const dim3 cg((unsigned)((words + 255) / 256));
     const int rounds = a.return_dist > 0.0f ? kMixedRounds : 1;
     for (int r = 0; r < rounds; ++r) {
-        a.from_returns = r > 0;
-        if (r > 0) {
-            hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, a.queue, 0);
-            int rc = check_launch("reset_counters_kernel");
+        const bool allow_return = r + 1 < rounds;
+        hipLaunchKernelGGL(clear_round_kernel, cg, dim3(256), 0, s, (uint32_t*)a.hflag, words, a.handoff_queue);
+        int rc = check_launch("clear_round_kernel");
+        if (rc) return rc;
+        timer.mark(kPassOther);
+        if (r == 0)
+            hipLaunchKernelGGL((trace_mixed_pass1_kernel<POSINV, false>), dim3(g1), dim3(kMixedBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((trace_mixed_pass1_kernel<POSINV, true>), dim3(g1r), dim3(kMixedBlock), 0, s, a);
+        rc = check_launch("trace_mixed_pass1_kernel");
+        if (rc) return rc;
+        timer.mark(kPassMarch);
+        if (allow_return) {
+            hipLaunchKernelGGL(clear_round_kernel, cg, dim3(256), 0, s, (uint32_t*)a.rflag, words, a.ret_queue);
+            rc = check_launch("clear_round_kernel");
             if (rc) return rc;
             timer.mark(kPassOther);
         }
-        a.replay = 1;
-        hipLaunchKernelGGL(trace_mixed_pass1_kernel<POSINV>, dim3(g1), dim3(kMixedBlock), 0, s, a);
-        a.replay = 0;
-        int rc = check_launch("trace_mixed_pass1_kernel");
-        if (rc) return rc;
-        timer.mark(kPassMarch);
-        hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, a.queue, 1);
-        rc = check_launch("reset_counters_kernel");
-        if (rc) return rc;
-        timer.mark(kPassOther);
-        a.allow_return = r + 1 < rounds;
-        hipLaunchKernelGGL(trace_mixed_pass2_kernel<POSINV>, dim3(g2), dim3(kPass2Block), 0, s, a);
+        if (allow_return)
+            hipLaunchKernelGGL((trace_mixed_pass2_kernel<POSINV, true>), dim3(g2), dim3(kPass2Block), 0, s, a);
+        else
+            hipLaunchKernelGGL((trace_mixed_pass2_kernel<POSINV, false>), dim3(g2), dim3(kPass2Block), 0, s, a);
         rc = check_launch("trace_mixed_pass2_kernel");
         if (rc) return rc;
         timer.mark(kPassFp32);
     }
-    hipLaunchKernelGGL(normals_list_kernel<POSINV>, dim3(g3), dim3(kMarchBlock), 0, s, a);
-    int rc = check_launch("normals_list_kernel");
-    if (rc) return rc;
-    timer.mark(kPassNormals);
+    if (a.normals) {
+        hipLaunchKernelGGL(normals_pass_kernel<POSINV>, dim3(g3), dim3(kMarchBlock), 0, s, a);
+        int rc = check_launch("normals_pass_kernel");
+        if (rc) return rc;
+        timer.mark(kPassNormals);
+    }
     return NGLOD_OK;
 }
 

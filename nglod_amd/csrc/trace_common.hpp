@@ -36,17 +36,9 @@ __device__ __forceinline__ void normalize_grad(float g0, float g1, float g2, flo
 constexpr int kMarchBlock = 768;
 constexpr int kQueues = 8;
 
-// PH_NQ: a finished hit whose normal is computed by a separate pass (the ray's
-// x, t, hit are already written; it waits to be appended to the normal list).
 // PH_REPLAY: the mixed-precision march stopped on a surface test; the kernel
 // rewinds the last steps and hands them to the FP32 pass.
-enum : int { PH_IDLE = -1, PH_INIT = 0, PH_MARCH = 1, PH_N0 = 2, PH_N5 = 7, PH_NQ = 8, PH_REPLAY = 9 };
-
-// A finished hit waiting for its normal (mixed form).  16 bytes.
-struct NormalItem {
-    int32_t ray;
-    float x0, x1, x2;
-};
+enum : int { PH_IDLE = -1, PH_INIT = 0, PH_MARCH = 1, PH_N0 = 2, PH_N5 = 7, PH_REPLAY = 9 };
 
 // State of a ray handed from the mixed-precision march to the FP32 pass:
 // either a pending evaluation at x (phase INIT or MARCH, re-evaluated in
@@ -75,23 +67,21 @@ struct TraceArgs {
     uint32_t* queue;
     unsigned long long* evals;  // [0] FP32-form evaluations, [1] mixed-form evaluations
     int32_t* ray_steps;
-    // Mixed form: rays handed from the mixed march to the FP32 pass.
-    int replay;        // 1 in the mixed march: a surface stop becomes PH_REPLAY
     CoarsePlan coarse; // leading levels the march kernels stage in LDS
+    // Mixed form.  Records passed between the passes are indexed by ray
+    // (record[ray], flag[ray] = 1 when written): no shared append counter, and
+    // a pass that reads them claims consecutive ray ids.
     int replay_steps;  // at most this many evaluations rewound and re-evaluated in FP32
     float near_dist;   // ... back to the last step of at least this length (0: all replay_steps)
-    struct Handoff* handoff;
-    uint32_t* handoff_count;
-    uint32_t* handoff_queue;
-    struct Handoff* ret;     // rays handed back from FP32 to the mixed march
-    uint32_t* ret_count;
-    uint32_t* ret_queue;
-    int from_returns;        // pass 1 refills from `ret` instead of the ray queues
-    int allow_return;        // pass 2 may hand rays back (not in the last round)
-    float return_dist;       // FP32 step length above which a ray goes back to mixed
-    int defer_normals;  // 1: finished hits go to the normal list instead of in-lane taps
-    NormalItem* nlist;
-    uint32_t* ncount;
+    struct Handoff* handoff;  // rays handed from the mixed march to the FP32 pass
+    uint8_t* hflag;
+    uint32_t* handoff_queue;  // [kQueues] claim counters of the FP32 pass
+    struct Handoff* ret;      // rays handed back from the FP32 pass to the mixed march
+    uint8_t* rflag;
+    uint32_t* ret_queue;      // [kQueues] claim counters of a pass 1 that resumes them
+    uint32_t* normal_queue;   // [kQueues] claim counters of the normals pass
+    float return_dist;        // FP32 step length above which a ray goes back to mixed
+    int defer_normals;        // 1: a finished hit is written out without its normal (a later pass computes it)
     int64_t qbeg[kQueues];
     int64_t qend[kQueues];
 };
@@ -134,7 +124,8 @@ struct RayState {
             a.t[ray] = t;
             a.hit[ray] = 1;
             if (a.ray_steps) a.ray_steps[ray] = steps;
-            phase = PH_NQ;
+            phase = PH_IDLE;
+            ray = -1;
         } else if (hit && a.normals) {
             phase = PH_N0;
             qx = x0 + kFdH;
@@ -145,8 +136,10 @@ struct RayState {
         }
     }
 
-    // Start of loop iteration `it` (SphereTracer.py:84-102).
-    __device__ __forceinline__ void check(const TraceArgs& a) {
+    // Start of loop iteration `it` (SphereTracer.py:84-102).  With `replay`
+    // (the mixed-precision march) a surface stop becomes PH_REPLAY instead of
+    // finishing the ray.
+    __device__ __forceinline__ void check(const TraceArgs& a, bool replay) {
         if (it >= a.num_steps) {
             finish(a);
             return;
@@ -154,7 +147,7 @@ struct RayState {
         hitflag = fabsf(t) < a.far_clip;
         const bool cond = (fabsf(D) > a.min_dis) && (fabsf((D + Dp) * 0.5f) > a.min_dis3) && hitflag;
         if (!cond) {
-            if (a.replay && hitflag) {
+            if (replay && hitflag) {
                 phase = PH_REPLAY;
                 return;
             }
@@ -236,20 +229,8 @@ struct RayState {
         }
     }
 
-    __device__ __forceinline__ void queue_normal(const TraceArgs& a, uint32_t slot) {
-        NormalItem it_;
-        it_.ray = (int32_t)ray;
-        it_.x0 = x0;
-        it_.x1 = x1;
-        it_.x2 = x2;
-        a.nlist[slot] = it_;
-        phase = PH_IDLE;
-        ray = -1;
-    }
-
-    __device__ __forceinline__ void hand_off(const TraceArgs& a, uint32_t slot) { hand_off_to(a.handoff, slot); }
-
-    __device__ __forceinline__ void hand_off_to(Handoff* list, uint32_t slot) {
+    // Write this ray's record at list[ray], set flag[ray] and free the lane.
+    __device__ __forceinline__ void hand_off_to(Handoff* list, uint8_t* flag) {
         Handoff h;
         h.ray = (int32_t)ray;
         h.phase = phase;
@@ -263,24 +244,25 @@ struct RayState {
         h.x2 = x2;
         h.hitflag = hitflag ? 1 : 0;
         h.pad = 0;
-        list[slot] = h;
+        list[ray] = h;
+        flag[ray] = 1;
         phase = PH_IDLE;
         ray = -1;
     }
 
-    __device__ __forceinline__ void consume(const TraceArgs& a, float d) {
+    __device__ __forceinline__ void consume(const TraceArgs& a, float d, bool replay = false) {
         if (phase == PH_INIT) {
             D = d;  // d = net(x) is not scaled by step_size (SphereTracer.py:63)
             Dp = D;
             it = 0;
-            check(a);
+            check(a, replay);
         } else if (phase == PH_MARCH) {
             Dp = D;
             D = d * a.step;
             t = t + D;
             ++it;
             ++steps;
-            check(a);
+            check(a, replay);
         } else if (phase < PH_N0 || phase > PH_N5) {
             // not an evaluating phase: end the ray rather than loop on it
             finish(a);
@@ -310,6 +292,80 @@ struct RayState {
                     ++phase;
                 }
             }
+        }
+    }
+};
+
+// Hands ray ids to the lanes of one wave.  The frame is split into kQueues
+// contiguous ranges with one claim counter each (counters[0..kQueues-1]);
+// blockIdx % kQueues picks the wave's own range first (one per XCD), and a
+// wave that drains it moves on to the next.  A wave claims CHUNK consecutive
+// ids per atomic (CHUNK = 0: as many as it has lanes asking) and hands them
+// out in lane order, so its rays stay neighbours in the frame.  With a flag
+// array, ids whose flag is 0 are skipped (the passes over ray-indexed records).
+struct RayFeed {
+    int q, tried;
+    bool drained;    // no ids left to claim (wave-uniform)
+    int64_t cb, ce;  // the wave's unused ids of its current chunk (wave-uniform)
+
+    __device__ __forceinline__ void init() {
+        q = blockIdx.x % kQueues;
+        tried = 0;
+        drained = false;
+        cb = ce = 0;
+    }
+
+    __device__ __forceinline__ bool empty() const { return drained && cb >= ce; }
+
+    // The next id for each lane with `need` (-1: none left).  SPLIT = false:
+    // the whole frame is one range with counter counters[0].
+    template <int CHUNK, bool SPLIT = true>
+    __device__ __forceinline__ int64_t next(uint32_t* counters, const TraceArgs& a, const uint8_t* flag,
+                                            uint32_t lane, bool need) {
+        int64_t id = -1;
+        for (;;) {
+            const bool want = need && id < 0 && !empty();
+            const uint64_t mask = __ballot(want);
+            if (mask == 0) break;
+            const int cnt = __popcll(mask);
+            if (cb >= ce) {
+                const int take = CHUNK > 0 ? CHUNK : cnt;
+                const int leader = __ffsll((unsigned long long)mask) - 1;
+                uint32_t base = 0;
+                const int qi = SPLIT ? q : 0;
+                if ((int)lane == leader) base = atomicAdd(&counters[qi], (uint32_t)take);
+                base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+                const int64_t lo = SPLIT ? a.qbeg[qi] : 0;
+                const int64_t qsize = (SPLIT ? a.qend[qi] : a.qend[kQueues - 1]) - lo;
+                if ((int64_t)base < qsize) {
+                    cb = lo + base;
+                    ce = lo + ((int64_t)base + take < qsize ? (int64_t)base + take : qsize);
+                }
+                if ((int64_t)base + take >= qsize) {
+                    q = (q + 1) % kQueues;
+                    if (!SPLIT || ++tried == kQueues) drained = true;
+                }
+                if (cb >= ce) continue;
+            }
+            if (want) {
+                const int64_t c = cb + (int64_t)mask_rank(mask);
+                if (c < ce && (!flag || flag[c])) id = c;
+            }
+            cb = cb + cnt < ce ? cb + cnt : ce;
+        }
+        return id;
+    }
+
+    // Start a ray in every idle lane, until each lane is busy or the rays run
+    // out.  A ray that misses the box, or starts inside it, finishes at once
+    // and its lane takes the next one.
+    template <int CHUNK>
+    __device__ __forceinline__ void refill(RayState& s, const TraceArgs& a, uint32_t lane) {
+        for (;;) {
+            const bool need = s.phase == PH_IDLE;
+            if (__ballot(need && !empty()) == 0) break;
+            const int64_t id = next<CHUNK>(a.queue, a, nullptr, lane, need);
+            if (id >= 0) s.start(a, id);
         }
     }
 };
