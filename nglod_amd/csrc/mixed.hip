@@ -37,11 +37,7 @@ constexpr int kMixedBlock = 256;
 #define NGLOD_PASS1_CHUNK 64
 #endif
 constexpr int kPass1Chunk = NGLOD_PASS1_CHUNK;
-// Passes over ray-indexed records claim 64 ids (about 40 records) per atomic.
-#ifndef NGLOD_CLAIM_CHUNK
-#define NGLOD_CLAIM_CHUNK 64
-#endif
-constexpr int kClaimChunk = NGLOD_CLAIM_CHUNK;
+
 constexpr int kPass2Block = 256;
 
 // ---------------------------------------------------------------------------
@@ -175,7 +171,7 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
     hist.clear();
     for (;;) {
         if (RESUME) {
-            const int64_t id = feed.next<kClaimChunk>(a.ret_queue, a, a.rflag, lane, s.phase == PH_IDLE);
+            const int64_t id = feed.next_flagged<true>(a.ret_queue, a, a.rflag, lane, s.phase == PH_IDLE);
             if (id >= 0) {
                 s.resume(a, a.ret[id]);
                 hist.clear();
@@ -215,7 +211,7 @@ __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArg
     s.phase = PH_IDLE;
     uint32_t nevals = 0;
     for (;;) {
-        const int64_t id = feed.next<kClaimChunk>(a.handoff_queue, a, a.hflag, lane, s.phase == PH_IDLE);
+        const int64_t id = feed.next_flagged<true>(a.handoff_queue, a, a.hflag, lane, s.phase == PH_IDLE);
         if (id >= 0) s.resume(a, a.handoff[id]);
 #ifdef NGLOD_DIAG_PASS2
         if (id >= 0) s.steps = 1000;
@@ -234,6 +230,30 @@ __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArg
     add_evals(a, 0, nevals, lane);
 }
 
+// Every lane claims the next id in [0, total) whose flag is set, from one
+// counter (one atomic per wave per round, as many ids as lanes still
+// looking); -1 once the ids run out (`drained`, wave-uniform).
+__device__ __forceinline__ int64_t claim_hits(uint32_t* counter, const uint8_t* flag, uint32_t total, uint32_t lane,
+                                              bool& drained) {
+    int64_t id = -1;
+    for (;;) {
+        const bool want = id < 0 && !drained;
+        const uint64_t mask = __ballot(want);
+        if (mask == 0) break;
+        const int cnt = __popcll(mask);
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)cnt);
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+        if (want) {
+            const uint32_t idx = base + mask_rank(mask);
+            if (idx < total && flag[idx]) id = idx;
+        }
+        if (base + (uint32_t)cnt >= total) drained = true;
+    }
+    return id;
+}
+
 // Pass 3: FP32 central-difference normals of the hits (hit[ray] = 1, the hit
 // point in x[ray]), one hit per lane, six evaluations each; waves claim
 // consecutive rays, so their hits are neighbours in the frame.
@@ -246,12 +266,12 @@ __global__ __launch_bounds__(kMarchBlock) void normals_pass_kernel(TraceArgs a) 
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
-    RayFeed feed;
-    feed.init();
+    const uint32_t total = (uint32_t)a.qend[kQueues - 1];
+    bool drained = false;
     uint32_t nevals = 0;
     for (;;) {
         // uniform work per hit: one range in frame order, a claim per idle lane
-        const int64_t id = feed.next<0, false>(a.normal_queue, a, a.hit, lane, true);
+        const int64_t id = claim_hits(a.normal_queue, a.hit, total, lane, drained);
         const bool valid = id >= 0;
         if (__ballot(valid) == 0) break;
         float p[3] = {0.0f, 0.0f, 0.0f};

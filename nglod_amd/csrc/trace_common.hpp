@@ -303,19 +303,92 @@ struct RayState {
 // ids per atomic (CHUNK = 0: as many as it has lanes asking) and hands them
 // out in lane order, so its rays stay neighbours in the frame.  With a flag
 // array, ids whose flag is 0 are skipped (the passes over ray-indexed records).
+// Position of the r-th (0-based) set bit of m; r < popcount(m).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __popcll((m >> pos) & ((1ull << w) - 1ull));
+        if (r >= c) {
+            r -= c;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
 struct RayFeed {
     int q, tried;
     bool drained;    // no ids left to claim (wave-uniform)
     int64_t cb, ce;  // the wave's unused ids of its current chunk (wave-uniform)
+    uint64_t fmask;  // next_flagged: flagged ids cb + bit not yet handed out (wave-uniform)
 
     __device__ __forceinline__ void init() {
         q = blockIdx.x % kQueues;
         tried = 0;
         drained = false;
         cb = ce = 0;
+        fmask = 0;
     }
 
     __device__ __forceinline__ bool empty() const { return drained && cb >= ce; }
+
+    // Claims the next chunk of `take` ids into [cb, ce); false if this claim
+    // found the range empty (the caller tries again unless drained).
+    template <bool SPLIT>
+    __device__ __forceinline__ bool claim(uint32_t* counters, const TraceArgs& a, uint32_t lane, int leader,
+                                          int take) {
+        const int qi = SPLIT ? q : 0;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counters[qi], (uint32_t)take);
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+        const int64_t lo = SPLIT ? a.qbeg[qi] : 0;
+        const int64_t qsize = (SPLIT ? a.qend[qi] : a.qend[kQueues - 1]) - lo;
+        const bool got = (int64_t)base < qsize;
+        if (got) {
+            cb = lo + base;
+            ce = lo + ((int64_t)base + take < qsize ? (int64_t)base + take : qsize);
+        }
+        if ((int64_t)base + take >= qsize) {
+            q = (q + 1) % kQueues;
+            if (!SPLIT || ++tried == kQueues) drained = true;
+        }
+        return got;
+    }
+
+    // The next id whose flag is set, for each lane with `need` (-1: none
+    // left).  The wave claims 64 ids at a time, loads their 64 flags in one
+    // coalesced load (a flag per lane) and hands the flagged ids out in order
+    // from the ballot mask.
+    template <bool SPLIT>
+    __device__ __forceinline__ int64_t next_flagged(uint32_t* counters, const TraceArgs& a, const uint8_t* flag,
+                                                    uint32_t lane, bool need) {
+        int64_t id = -1;
+        for (;;) {
+            const bool want = need && id < 0;
+            const uint64_t mask = __ballot(want);
+            if (mask == 0) break;
+            if (fmask == 0) {
+                if (drained) break;
+                const int leader = __ffsll((unsigned long long)mask) - 1;
+                if (!claim<SPLIT>(counters, a, lane, leader, 64)) continue;
+                const int64_t c = cb + (int64_t)lane;
+                fmask = __ballot(c < ce && flag[c] != 0);
+                if (fmask == 0) continue;
+            }
+            const int avail = __popcll(fmask);
+            const int cnt = __popcll(mask);
+            if (want) {
+                const int r = (int)mask_rank(mask);
+                if (r < avail) id = cb + nth_set_bit(fmask, r);
+            }
+            if (cnt >= avail)
+                fmask = 0;
+            else
+                fmask &= ~((1ull << nth_set_bit(fmask, cnt)) - 1ull);
+        }
+        return id;
+    }
 
     // The next id for each lane with `need` (-1: none left).  SPLIT = false:
     // the whole frame is one range with counter counters[0].
