@@ -124,8 +124,9 @@ int nglod_sdf_fwd(const nglod_model_t* model, const float* x, int64_t n, int32_t
 int nglod_sdf_fwd_all(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
                       float* out, void* stream);
 
-/* Workspace bytes nglod_trace_dense needs for n rays in `form` (ray queue
- * counters; for NGLOD_MIXED also the FP32 hand-off list, 48 B per ray). */
+/* Workspace bytes nglod_trace_dense needs for n rays in `form` (256 bytes of
+ * claim counters; for NGLOD_MIXED also the ray-indexed hand-off and hand-back
+ * records, 48 B each, and their flags, 1 B each, per ray). */
 size_t nglod_trace_dense_workspace_size(int64_t n, int32_t form);
 
 /* Persistent sphere tracer with SphereTracer semantics

@@ -126,5 +126,7 @@ struct TraceArgs;
 // Mixed-precision launchers (mixed.hip).
 int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out, hipStream_t s);
 int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& timer);
+// FP32 normals of every hit of a finished trace (hit[], x[] written).
+int launch_normals_pass(const TraceArgs& a, int64_t n, hipStream_t s);
 
 }  // namespace nglod

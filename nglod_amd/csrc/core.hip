@@ -189,12 +189,6 @@ static int persistent_grid(K kernel, int64_t n, int block) {
     return (int)(want < cap ? (want < 1 ? 1 : want) : cap);
 }
 
-void split_queues(int64_t n, int64_t* beg, int64_t* end) {
-    for (int i = 0; i < kQueues; ++i) {
-        beg[i] = n * i / kQueues;
-        end[i] = n * (i + 1) / kQueues;
-    }
-}
 
 }  // namespace nglod
 
@@ -280,12 +274,11 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
     return check_launch("normals_fp32_kernel");
 }
 
-// Workspace: 256 bytes of counters (the ray queues, then the claim counters
-// of pass 2, of a pass 1 over returned rays and of the normals pass, kQueues
-// each); for the
-// mixed form then hand-off records [n], returned records [n], hand-off flags
+// Workspace: kWsCounters bytes of counters (the ray queues, then the claim
+// counters of pass 2, of a pass 1 over returned rays and of the normals pass,
+// kQueues each); for the mixed form then hand-off records [n], returned records [n], hand-off flags
 // [n] and return flags [n] (each flag array padded to 256 bytes).
-static constexpr size_t kWsCounters = 256;
+static constexpr size_t kWsCounters = (16 * kQueues + 255) / 256 * 256;
 static constexpr int kCounterWords = 4 * kQueues;
 
 static size_t flag_bytes(int64_t n) { return ((size_t)n + 255) / 256 * 256; }
@@ -346,15 +339,19 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
         a.rflag = a.hflag + flag_bytes(n);
     }
     a.return_dist = (float)params->return_dist;
-    a.defer_normals = form == NGLOD_MIXED ? 1 : 0;
+#ifndef NGLOD_FP32_NORMALS_PASS
+#define NGLOD_FP32_NORMALS_PASS 0
+#endif
+    const bool fp32_normals_pass = NGLOD_FP32_NORMALS_PASS && form == NGLOD_FP32 && params->normals;
+    a.defer_normals = (form == NGLOD_MIXED || fp32_normals_pass) ? 1 : 0;
     a.coarse = coarse_plan(m, lod);
     a.replay_steps = params->refine_steps < 1 ? 1 : params->refine_steps;
     a.near_dist = (float)params->near_dist;
     a.evals = (unsigned long long*)evals_out;
     a.ray_steps = ray_steps;
-    split_queues(n, a.qbeg, a.qend);
+    a.nrays = n;
     PassTimer timer(params->pass_ms, kPassSlots, s);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, a.queue, kCounterWords, 0u);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((kCounterWords + 63) / 64), dim3(64), 0, s, a.queue, kCounterWords, 0u);
     rc = check_launch("fill_u32_kernel");
     if (rc) return rc;
     timer.mark(kPassOther);
@@ -372,6 +369,11 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     rc = check_launch("trace_dense_fp32_kernel");
     if (rc) return rc;
     timer.mark(kPassMarch);
+    if (fp32_normals_pass) {
+        rc = launch_normals_pass(a, n, s);
+        if (rc) return rc;
+        timer.mark(kPassNormals);
+    }
     return timer.finish();
 }
 

@@ -28,17 +28,15 @@
 namespace nglod {
 
 // Passes 1 and 2 read every level from global memory: with LDS copies of the
-// coarse levels (FP16 or FP32 layout) they measured slower (pass 1, by 10-20%)
-// or the same (pass 2) on the 1280x720 LOD-4 frame.
+// coarse levels (FP16 or FP32 layout) they measured 10-25% slower on the
+// 1280x720 LOD-4 frame.  Both run 256-thread blocks at the occupancy their
+// registers allow; capping registers for more waves per SIMD, or fewer,
+// measured slower.
 constexpr int kMixedBlock = 256;
+constexpr int kPass2Block = 256;
 // Pass 1 claims rays 64 ids at a time (the FP32 march claims per idle lane):
 // measured faster for the mixed march, slightly slower for the FP32 one.
-#ifndef NGLOD_PASS1_CHUNK
-#define NGLOD_PASS1_CHUNK 64
-#endif
-constexpr int kPass1Chunk = NGLOD_PASS1_CHUNK;
-
-constexpr int kPass2Block = 256;
+constexpr int kPass1Chunk = 64;
 
 // ---------------------------------------------------------------------------
 __global__ void pack_grid_kernel(const float* __restrict__ src, __half* __restrict__ dst, int64_t n) {
@@ -213,9 +211,6 @@ __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArg
     for (;;) {
         const int64_t id = feed.next_flagged<true>(a.handoff_queue, a, a.hflag, lane, s.phase == PH_IDLE);
         if (id >= 0) s.resume(a, a.handoff[id]);
-#ifdef NGLOD_DIAG_PASS2
-        if (id >= 0) s.steps = 1000;
-#endif
         if (__ballot(s.phase != PH_IDLE) == 0) break;
         const bool busy = s.phase != PH_IDLE;
         const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
@@ -266,7 +261,7 @@ __global__ __launch_bounds__(kMarchBlock) void normals_pass_kernel(TraceArgs a) 
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
-    const uint32_t total = (uint32_t)a.qend[kQueues - 1];
+    const uint32_t total = (uint32_t)a.nrays;
     bool drained = false;
     uint32_t nevals = 0;
     for (;;) {
@@ -381,6 +376,16 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
 
 int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& timer) {
     return a.m.pos_inv ? trace_mixed_rounds<true>(a, n, s, timer) : trace_mixed_rounds<false>(a, n, s, timer);
+}
+
+int launch_normals_pass(const TraceArgs& a, int64_t n, hipStream_t s) {
+    if (a.m.pos_inv)
+        hipLaunchKernelGGL(normals_pass_kernel<true>, dim3(grid_for(normals_pass_kernel<true>, n, kMarchBlock)),
+                           dim3(kMarchBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(normals_pass_kernel<false>, dim3(grid_for(normals_pass_kernel<false>, n, kMarchBlock)),
+                           dim3(kMarchBlock), 0, s, a);
+    return check_launch("normals_pass_kernel");
 }
 
 }  // namespace nglod

@@ -31,10 +31,14 @@ __device__ __forceinline__ void normalize_grad(float g0, float g1, float g2, flo
 // (workgroups are dispatched to XCDs round-robin, so blockIdx % 8 picks the
 // XCD-local range and its L2 sees one band of the image); a wave that drains
 // its own range moves on to the next ones.
-// Threads per block of the march kernels: one block of 12 waves per CU, all
-// sharing the decoder and coarse-level tables in LDS.
-constexpr int kMarchBlock = 768;
-constexpr int kQueues = 8;
+// Threads per block of the FP32 march and the normals pass: one block of 8
+// waves per CU (the 128 KB of decoder and coarse-level tables in LDS leave room
+// for one), sharing those tables.  Measured on the 1280x720 LOD-4 frame: 512
+// threads 4.66 ms, 768 4.99, 384 5.10, 1024 5.36, 256 5.68 (more waves per CU
+// evict each other's cells from the L1).
+constexpr int kMarchBlock = 512;
+constexpr int kQueues = 8;  // a multiple of 8 (one share per XCD)
+static_assert(kQueues % 8 == 0 && kQueues <= 256, "kQueues: a multiple of 8, at most 256");
 
 // PH_REPLAY: the mixed-precision march stopped on a surface test; the kernel
 // rewinds the last steps and hands them to the FP32 pass.
@@ -82,8 +86,7 @@ struct TraceArgs {
     uint32_t* normal_queue;   // [kQueues] claim counters of the normals pass
     float return_dist;        // FP32 step length above which a ray goes back to mixed
     int defer_normals;        // 1: a finished hit is written out without its normal (a later pass computes it)
-    int64_t qbeg[kQueues];
-    int64_t qend[kQueues];
+    int64_t nrays;  // the frame is split into kQueues ranges [nrays*q/kQueues, nrays*(q+1)/kQueues)
 };
 
 struct RayState {
@@ -196,7 +199,7 @@ struct RayState {
     // record whose ray index is out of range (never the case for a written
     // record) leaves the lane idle instead of indexing outside the arrays.
     __device__ __forceinline__ void resume(const TraceArgs& a, const Handoff& h) {
-        if (h.ray < 0 || (int64_t)h.ray >= a.qend[kQueues - 1]) {
+        if (h.ray < 0 || (int64_t)h.ray >= a.nrays) {
             phase = PH_IDLE;
             ray = -1;
             return;
@@ -317,14 +320,16 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
     return pos;
 }
 
+__device__ __forceinline__ int64_t range_begin(const TraceArgs& a, int q) { return a.nrays * q / kQueues; }
+
 struct RayFeed {
-    int q, tried;
+    int q0, q, tried;
     bool drained;    // no ids left to claim (wave-uniform)
     int64_t cb, ce;  // the wave's unused ids of its current chunk (wave-uniform)
     uint64_t fmask;  // next_flagged: flagged ids cb + bit not yet handed out (wave-uniform)
 
     __device__ __forceinline__ void init() {
-        q = blockIdx.x % kQueues;
+        q0 = q = blockIdx.x % kQueues;
         tried = 0;
         drained = false;
         cb = ce = 0;
@@ -332,6 +337,14 @@ struct RayFeed {
     }
 
     __device__ __forceinline__ bool empty() const { return drained && cb >= ce; }
+
+    // The next range to claim from: first the other ranges of this XCD
+    // (q0 + 8j), then those of the next XCDs; each range once.
+    __device__ __forceinline__ void advance() {
+        constexpr int m = kQueues / 8;
+        ++tried;
+        q = (q0 + 8 * (tried % m) + tried / m) % kQueues;
+    }
 
     // Claims the next chunk of `take` ids into [cb, ce); false if this claim
     // found the range empty (the caller tries again unless drained).
@@ -342,16 +355,16 @@ struct RayFeed {
         uint32_t base = 0;
         if ((int)lane == leader) base = atomicAdd(&counters[qi], (uint32_t)take);
         base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-        const int64_t lo = SPLIT ? a.qbeg[qi] : 0;
-        const int64_t qsize = (SPLIT ? a.qend[qi] : a.qend[kQueues - 1]) - lo;
+        const int64_t lo = SPLIT ? range_begin(a, qi) : 0;
+        const int64_t qsize = (SPLIT ? range_begin(a, qi + 1) : a.nrays) - lo;
         const bool got = (int64_t)base < qsize;
         if (got) {
             cb = lo + base;
             ce = lo + ((int64_t)base + take < qsize ? (int64_t)base + take : qsize);
         }
         if ((int64_t)base + take >= qsize) {
-            q = (q + 1) % kQueues;
-            if (!SPLIT || ++tried == kQueues) drained = true;
+            advance();
+            if (!SPLIT || tried == kQueues) drained = true;
         }
         return got;
     }
@@ -408,15 +421,15 @@ struct RayFeed {
                 const int qi = SPLIT ? q : 0;
                 if ((int)lane == leader) base = atomicAdd(&counters[qi], (uint32_t)take);
                 base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-                const int64_t lo = SPLIT ? a.qbeg[qi] : 0;
-                const int64_t qsize = (SPLIT ? a.qend[qi] : a.qend[kQueues - 1]) - lo;
+                const int64_t lo = SPLIT ? range_begin(a, qi) : 0;
+                const int64_t qsize = (SPLIT ? range_begin(a, qi + 1) : a.nrays) - lo;
                 if ((int64_t)base < qsize) {
                     cb = lo + base;
                     ce = lo + ((int64_t)base + take < qsize ? (int64_t)base + take : qsize);
                 }
                 if ((int64_t)base + take >= qsize) {
-                    q = (q + 1) % kQueues;
-                    if (!SPLIT || ++tried == kQueues) drained = true;
+                    advance();
+                    if (!SPLIT || tried == kQueues) drained = true;
                 }
                 if (cb >= ce) continue;
             }
