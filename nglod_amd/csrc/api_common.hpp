@@ -91,12 +91,15 @@ class PassTimer {
         mark(-1);
     }
     ~PassTimer() {
-        for (int i = 0; i < n_; ++i) hipEventDestroy(ev_[i]);
+        for (int i = 0; i < n_; ++i) (void)hipEventDestroy(ev_[i]);
     }
     void mark(int slot) {
         if (!out_ || n_ >= kMax) return;
         if (hipEventCreate(&ev_[n_]) != hipSuccess) return;
-        hipEventRecord(ev_[n_], s_);
+        if (hipEventRecord(ev_[n_], s_) != hipSuccess) {
+            (void)hipEventDestroy(ev_[n_]);
+            return;
+        }
         slot_[n_++] = slot;
     }
     int finish() {
@@ -126,7 +129,5 @@ struct TraceArgs;
 // Mixed-precision launchers (mixed.hip).
 int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out, hipStream_t s);
 int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& timer);
-// FP32 normals of every hit of a finished trace (hit[], x[] written).
-int launch_normals_pass(const TraceArgs& a, int64_t n, hipStream_t s);
 
 }  // namespace nglod
