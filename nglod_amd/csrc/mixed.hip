@@ -378,16 +378,6 @@ int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& 
     return a.m.pos_inv ? trace_mixed_rounds<true>(a, n, s, timer) : trace_mixed_rounds<false>(a, n, s, timer);
 }
 
-int launch_normals_pass(const TraceArgs& a, int64_t n, hipStream_t s) {
-    if (a.m.pos_inv)
-        hipLaunchKernelGGL(normals_pass_kernel<true>, dim3(grid_for(normals_pass_kernel<true>, n, kMarchBlock)),
-                           dim3(kMarchBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL(normals_pass_kernel<false>, dim3(grid_for(normals_pass_kernel<false>, n, kMarchBlock)),
-                           dim3(kMarchBlock), 0, s, a);
-    return check_launch("normals_pass_kernel");
-}
-
 }  // namespace nglod
 
 using namespace nglod;
