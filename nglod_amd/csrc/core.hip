@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kMarchBlock) void trace_dense_fp32_kernel(TraceArgs
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
     RayFeed feed;
-    feed.init();
+    feed.init(a.nrays);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;

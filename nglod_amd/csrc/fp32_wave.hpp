@@ -31,12 +31,12 @@ struct Fp32Lds {
     float wxyz[kH][4];  // W0 columns 0..2 (xyz input); [3] unused
 };
 
-// Stage decoder `dec` into LDS.  Every thread of the block must call this,
-// followed by __syncthreads().
-__device__ __forceinline__ void stage_fp32_lds(const ModelArgs& m, int dec, Fp32Lds* s) {
-    const int din = m.pos_inv ? kC : kC + 3;
-    const int off = m.pos_inv ? 0 : 3;
-    const float* W0 = m.w0[dec];
+// Stage a decoder (W0 [128][din], din = 32 or 35 with xyz first; b0; w1) into
+// LDS.  Every thread of the block must call this, followed by __syncthreads().
+__device__ __forceinline__ void stage_fp32_lds_raw(const float* W0, int din, const float* b0, const float* w1,
+                                                   Fp32Lds* s) {
+    const bool pos_inv = din == kC;
+    const int off = pos_inv ? 0 : 3;
     for (int j = threadIdx.x; j < 4 * 4 * 64; j += blockDim.x) {
         const int lane = j & 63;
         const int kq = (j >> 6) & 3;
@@ -51,9 +51,9 @@ __device__ __forceinline__ void stage_fp32_lds(const ModelArgs& m, int dec, Fp32
         s->afrag[j] = v;
     }
     for (int j = threadIdx.x; j < kH; j += blockDim.x) {
-        s->b0[j] = m.b0[dec][j];
-        s->w1[j] = m.w1[dec][j];
-        if (!m.pos_inv) {
+        s->b0[j] = b0[j];
+        s->w1[j] = w1[j];
+        if (!pos_inv) {
             s->wxyz[j][0] = W0[j * din + 0];
             s->wxyz[j][1] = W0[j * din + 1];
             s->wxyz[j][2] = W0[j * din + 2];
@@ -62,6 +62,11 @@ __device__ __forceinline__ void stage_fp32_lds(const ModelArgs& m, int dec, Fp32
         }
         s->wxyz[j][3] = 0.0f;
     }
+}
+
+// Stage decoder `dec` of the model.
+__device__ __forceinline__ void stage_fp32_lds(const ModelArgs& m, int dec, Fp32Lds* s) {
+    stage_fp32_lds_raw(m.w0[dec], m.pos_inv ? kC : kC + 3, m.b0[dec], m.w1[dec], s);
 }
 
 // Second layer over one 32x32 output tile held in registers: sum of
@@ -100,20 +105,11 @@ __device__ __forceinline__ float layer2_tile(const f32x16_t& c, int h, uint32_t 
 // and a 32-query tile with no active lane skips its MFMAs.  Each output
 // column depends only on its own query, so a lane's result is the same
 // whatever the other lanes hold.
+// The decoder on the 64 lanes' feature vectors f (zero on inactive lanes).
+// Wave-wide, as sdf_fp32_wave.
 template <bool POSINV>
-__device__ __forceinline__ float sdf_fp32_wave(const ModelArgs& m, int lod, const Fp32Lds* s, float b1,
-                                               uint32_t lane, bool active, float px, float py, float pz,
-                                               CoarsePlan cp = CoarsePlan{0, 0}, const CoarseLds32* cs = nullptr) {
-    float f[kC];
-    if (active) {
-        if (cp.levels > 0)
-            gather_fp32_coarse(m, lod, cp, cs, px, py, pz, f);
-        else
-            gather_fp32(m, lod, px, py, pz, f);
-    } else {
-#pragma unroll
-        for (int c = 0; c < kC; ++c) f[c] = 0.0f;
-    }
+__device__ __forceinline__ float decode_fp32_wave(const float (&f)[kC], const Fp32Lds* s, float b1, uint32_t lane,
+                                                  bool active, float px, float py, float pz) {
     const uint64_t am = __ballot(active);
     const bool use0 = (uint32_t)am != 0u, use1 = (uint32_t)(am >> 32) != 0u;
     const uint32_t half = lane >> 5;
@@ -164,6 +160,23 @@ __device__ __forceinline__ float sdf_fp32_wave(const ModelArgs& m, int lod, cons
     part0 += __shfl_xor(part0, 32);
     part1 += __shfl_xor(part1, 32);
     return (lo ? part0 : part1) + b1;
+}
+
+template <bool POSINV>
+__device__ __forceinline__ float sdf_fp32_wave(const ModelArgs& m, int lod, const Fp32Lds* s, float b1,
+                                               uint32_t lane, bool active, float px, float py, float pz,
+                                               CoarsePlan cp = CoarsePlan{0, 0}, const CoarseLds32* cs = nullptr) {
+    float f[kC];
+    if (active) {
+        if (cp.levels > 0)
+            gather_fp32_coarse(m, lod, cp, cs, px, py, pz, f);
+        else
+            gather_fp32(m, lod, px, py, pz, f);
+    } else {
+#pragma unroll
+        for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+    }
+    return decode_fp32_wave<POSINV>(f, s, b1, lane, active, px, py, pz);
 }
 
 }  // namespace nglod

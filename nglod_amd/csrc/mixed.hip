@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
     const uint32_t lane = lane_id();
     const float b1 = a.m.b1[a.dec][0];
     RayFeed feed;
-    feed.init();
+    feed.init(a.nrays);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
     hist.clear();
     for (;;) {
         if (RESUME) {
-            const int64_t id = feed.next_flagged<true>(a.ret_queue, a, a.rflag, lane, s.phase == PH_IDLE);
+            const int64_t id = feed.next_flagged<true>(a.ret_queue, a.rflag, lane, s.phase == PH_IDLE);
             if (id >= 0) {
                 s.resume(a, a.ret[id]);
                 hist.clear();
@@ -203,13 +203,13 @@ __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArg
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
     RayFeed feed;
-    feed.init();
+    feed.init(a.nrays);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
     uint32_t nevals = 0;
     for (;;) {
-        const int64_t id = feed.next_flagged<true>(a.handoff_queue, a, a.hflag, lane, s.phase == PH_IDLE);
+        const int64_t id = feed.next_flagged<true>(a.handoff_queue, a.hflag, lane, s.phase == PH_IDLE);
         if (id >= 0) s.resume(a, a.handoff[id]);
         if (__ballot(s.phase != PH_IDLE) == 0) break;
         const bool busy = s.phase != PH_IDLE;

@@ -18,6 +18,8 @@
 #include <rocprim/rocprim.hpp>
 
 #include "api_common.hpp"
+#include "fp32_wave.hpp"
+#include "trace_common.hpp"
 
 namespace nglod {
 
@@ -311,32 +313,46 @@ struct SolRay {
     }
 };
 
-__global__ __launch_bounds__(256) void trace_sparse_kernel(SolArgs a) {
+// Persistent sparse march: one ray per lane (SolRay), refilled from kQueues
+// ray ranges (RayFeed); the features come from the sparse corner table, the
+// decoder is the FP32 MFMA one of the dense form (fp32_wave.hpp), staged in
+// LDS once per block.  512-thread blocks, as the dense FP32 march.
+// 512 and 256 threads measured the same (3.23 / 3.25 ms on the 1280x720 LOD-4
+// frame), 768 slower (3.62 ms).
+constexpr int kSparseBlock = 512;
+
+template <bool POSINV>
+__global__ __launch_bounds__(kSparseBlock) void trace_sparse_kernel(SolArgs a) {
+    __shared__ Fp32Lds lds;
+    stage_fp32_lds_raw(a.w0, a.din, a.b0, a.w1, &lds);
+    __syncthreads();
+    const float b1 = a.b1[0];
     const uint32_t lane = lane_id();
-    bool drained = false;
+    RayFeed feed;
+    feed.init(a.n);
     SolRay s;
     s.ray = -1;
     s.phase = SP_IDLE;
     uint32_t nevals = 0;
     for (;;) {
+        // a ray without nuggets finishes at once and its lane takes the next
         for (;;) {
-            const bool need = (s.phase == SP_IDLE) && !drained;
-            const uint64_t mask = __ballot(need);
-            if (mask == 0) break;
-            const int cnt = __popcll(mask);
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(a.queue, (uint32_t)cnt);
-            base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-            if (need) {
-                const int64_t idx = (int64_t)base + (int64_t)mask_rank(mask);
-                if (idx < a.n) s.start(a, idx);
-            }
-            if ((int64_t)base + cnt >= a.n) drained = true;
+            const bool need = s.phase == SP_IDLE;
+            if (__ballot(need && !feed.empty()) == 0) break;
+            const int64_t id = feed.next<0>(a.queue, nullptr, lane, need);
+            if (id >= 0) s.start(a, id);
         }
         if (__ballot(s.phase != SP_IDLE) == 0) break;
-        if (s.phase != SP_IDLE) {
-            const float d = sol_eval(a, s.qx, s.qy, s.qz, s.pidx);
+        const bool busy = s.phase != SP_IDLE;
+        float f[kC];
+        if (busy) {
+            sol_features(a, s.qx, s.qy, s.qz, s.pidx + a.psum_target, f);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kC; ++k) f[k] = 0.0f;
+        }
+        const float d = decode_fp32_wave<POSINV>(f, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        if (busy) {
             ++nevals;
             s.consume(a, d);
         }
@@ -344,7 +360,7 @@ __global__ __launch_bounds__(256) void trace_sparse_kernel(SolArgs a) {
     if (a.evals) {
         unsigned long long tot = nevals;
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-        if (lane == 0) atomicAdd(a.evals, tot);
+        if (lane == 0 && tot) atomicAdd(a.evals, tot);
     }
 }
 
@@ -374,8 +390,9 @@ __global__ void point_codes_kernel(const float* __restrict__ x, const uint8_t* _
     codes[i] = keep ? (uint32_t)morton_u64(c[0], c[1], c[2]) : 0xFFFFFFFFu;
 }
 
-__global__ void fill_queue_kernel(uint32_t* q) {
-    if (threadIdx.x == 0) q[0] = 0;
+// Zeroes the first kQueues counters of a workspace.
+__global__ void clear_counters_kernel(uint32_t* q) {
+    if (threadIdx.x < kQueues) q[threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -561,8 +578,8 @@ int nglod_sol_trinkets(const int16_t* points, const int32_t* pyramid, int32_t le
     const int32_t* pcnt = pyramid;
     const int32_t* psum = pyramid + (level + 2);
     uint32_t* missing = (uint32_t*)workspace;
-    hipLaunchKernelGGL(fill_queue_kernel, dim3(1), dim3(64), 0, s, missing);
-    int rc = check_launch("fill_queue_kernel");
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, s, missing);
+    int rc = check_launch("clear_counters_kernel");
     if (rc) return rc;
     int64_t off = 0;
     for (int l = 2; l <= target; ++l) {
@@ -651,16 +668,22 @@ int nglod_trace_sparse(const nglod_sol_t* sol, const float* ray_o, const float* 
     a.iters = iters;
     a.queue = (uint32_t*)workspace;
     a.evals = (unsigned long long*)evals_out;
-    hipLaunchKernelGGL(fill_queue_kernel, dim3(1), dim3(64), 0, s, a.queue);
-    rc = check_launch("fill_queue_kernel");
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, s, a.queue);
+    rc = check_launch("clear_counters_kernel");
     if (rc) return rc;
+    const bool posinv = a.din == kC;
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_sparse_kernel, 256, 0) != hipSuccess || per_cu <= 0)
-        per_cu = 4;
-    const int64_t want = (n + 255) / 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, posinv ? trace_sparse_kernel<true> : trace_sparse_kernel<false>, kSparseBlock, 0) != hipSuccess ||
+        per_cu <= 0)
+        per_cu = 1;
+    const int64_t want = (n + kSparseBlock - 1) / kSparseBlock;
     const int64_t cap = (int64_t)device_cu_count() * per_cu;
     const int grid = (int)(want < cap ? want : cap);
-    hipLaunchKernelGGL(trace_sparse_kernel, dim3(grid), dim3(256), 0, s, a);
+    if (posinv)
+        hipLaunchKernelGGL(trace_sparse_kernel<true>, dim3(grid), dim3(kSparseBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL(trace_sparse_kernel<false>, dim3(grid), dim3(kSparseBlock), 0, s, a);
     return check_launch("trace_sparse_kernel");
 }
 

@@ -320,15 +320,15 @@ __device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
     return pos;
 }
 
-__device__ __forceinline__ int64_t range_begin(const TraceArgs& a, int q) { return a.nrays * q / kQueues; }
-
 struct RayFeed {
+    int64_t nrays;
     int q0, q, tried;
     bool drained;    // no ids left to claim (wave-uniform)
     int64_t cb, ce;  // the wave's unused ids of its current chunk (wave-uniform)
     uint64_t fmask;  // next_flagged: flagged ids cb + bit not yet handed out (wave-uniform)
 
-    __device__ __forceinline__ void init() {
+    __device__ __forceinline__ void init(int64_t n) {
+        nrays = n;
         q0 = q = blockIdx.x % kQueues;
         tried = 0;
         drained = false;
@@ -348,15 +348,16 @@ struct RayFeed {
 
     // Claims the next chunk of `take` ids into [cb, ce); false if this claim
     // found the range empty (the caller tries again unless drained).
+    __device__ __forceinline__ int64_t range_begin(int qi) const { return nrays * qi / kQueues; }
+
     template <bool SPLIT>
-    __device__ __forceinline__ bool claim(uint32_t* counters, const TraceArgs& a, uint32_t lane, int leader,
-                                          int take) {
+    __device__ __forceinline__ bool claim(uint32_t* counters, uint32_t lane, int leader, int take) {
         const int qi = SPLIT ? q : 0;
         uint32_t base = 0;
         if ((int)lane == leader) base = atomicAdd(&counters[qi], (uint32_t)take);
         base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-        const int64_t lo = SPLIT ? range_begin(a, qi) : 0;
-        const int64_t qsize = (SPLIT ? range_begin(a, qi + 1) : a.nrays) - lo;
+        const int64_t lo = SPLIT ? range_begin(qi) : 0;
+        const int64_t qsize = (SPLIT ? range_begin(qi + 1) : nrays) - lo;
         const bool got = (int64_t)base < qsize;
         if (got) {
             cb = lo + base;
@@ -374,8 +375,8 @@ struct RayFeed {
     // coalesced load (a flag per lane) and hands the flagged ids out in order
     // from the ballot mask.
     template <bool SPLIT>
-    __device__ __forceinline__ int64_t next_flagged(uint32_t* counters, const TraceArgs& a, const uint8_t* flag,
-                                                    uint32_t lane, bool need) {
+    __device__ __forceinline__ int64_t next_flagged(uint32_t* counters, const uint8_t* flag, uint32_t lane,
+                                                    bool need) {
         int64_t id = -1;
         for (;;) {
             const bool want = need && id < 0;
@@ -384,7 +385,7 @@ struct RayFeed {
             if (fmask == 0) {
                 if (drained) break;
                 const int leader = __ffsll((unsigned long long)mask) - 1;
-                if (!claim<SPLIT>(counters, a, lane, leader, 64)) continue;
+                if (!claim<SPLIT>(counters, lane, leader, 64)) continue;
                 const int64_t c = cb + (int64_t)lane;
                 fmask = __ballot(c < ce && flag[c] != 0);
                 if (fmask == 0) continue;
@@ -406,8 +407,7 @@ struct RayFeed {
     // The next id for each lane with `need` (-1: none left).  SPLIT = false:
     // the whole frame is one range with counter counters[0].
     template <int CHUNK, bool SPLIT = true>
-    __device__ __forceinline__ int64_t next(uint32_t* counters, const TraceArgs& a, const uint8_t* flag,
-                                            uint32_t lane, bool need) {
+    __device__ __forceinline__ int64_t next(uint32_t* counters, const uint8_t* flag, uint32_t lane, bool need) {
         int64_t id = -1;
         for (;;) {
             const bool want = need && id < 0 && !empty();
@@ -421,8 +421,8 @@ struct RayFeed {
                 const int qi = SPLIT ? q : 0;
                 if ((int)lane == leader) base = atomicAdd(&counters[qi], (uint32_t)take);
                 base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-                const int64_t lo = SPLIT ? range_begin(a, qi) : 0;
-                const int64_t qsize = (SPLIT ? range_begin(a, qi + 1) : a.nrays) - lo;
+                const int64_t lo = SPLIT ? range_begin(qi) : 0;
+                const int64_t qsize = (SPLIT ? range_begin(qi + 1) : nrays) - lo;
                 if ((int64_t)base < qsize) {
                     cb = lo + base;
                     ce = lo + ((int64_t)base + take < qsize ? (int64_t)base + take : qsize);
@@ -450,7 +450,7 @@ struct RayFeed {
         for (;;) {
             const bool need = s.phase == PH_IDLE;
             if (__ballot(need && !empty()) == 0) break;
-            const int64_t id = next<CHUNK>(a.queue, a, nullptr, lane, need);
+            const int64_t id = next<CHUNK>(a.queue, nullptr, lane, need);
             if (id >= 0) s.start(a, id);
         }
     }

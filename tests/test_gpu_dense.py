@@ -182,6 +182,29 @@ def test_trace_same_alone_and_in_the_mix(dev):
         assert torch.equal(a, b), name
 
 
+@pytest.mark.parametrize("form,return_dist", [("fp32", 0.0), ("mixed", 0.0), ("mixed", 1e-2)])
+def test_trace_ragged_counts(dev, form, return_dist):
+    """Ray counts below, at and just past the wave and claim sizes (some of
+    the 8 ray ranges empty, partial 64-id chunks), and none at all: every
+    ray's result equals its result in a full frame."""
+    net = workloads.make_sphere_model(radius=0.6, num_lods=5, device=dev, seed=8)
+    net.lod = 4
+    o, d = workloads.camera_rays(96, 96, device=dev)
+    kw = {} if form == "fp32" else {"return_dist": return_dist}
+    tracer = SphereTracer(num_steps=128, form=form, **kw)
+    full = tracer(net, o, d)
+    assert int(full.hit.sum()) > 1000
+    # a band of rays across the sphere's silhouette (hits, grazing rays and misses)
+    start = 48 * 96 + 10
+    for n in (0, 1, 7, 8, 9, 63, 64, 65, 127, 129, 1003):
+        part = tracer(net, o[start:start + n], d[start:start + n])
+        for name in ("x", "depth", "hit", "normal"):
+            a = getattr(full, name)[start:start + n]
+            b = getattr(part, name)
+            assert a.shape == b.shape, (n, name)
+            assert torch.equal(a, b), (n, name)
+
+
 def test_normals_fd_vs_oracle(dev):
     from nglod_amd import _lib
     net = workloads.make_model(num_lods=5, seed=30, device="cpu")
