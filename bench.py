@@ -120,6 +120,33 @@ def sparse_frame(args, net, o, d, dev):
             "note": "raytrace_ms includes the host read-back of the nugget count (a setup-style synchronous call)"}
 
 
+def query_bench(net, dev, lod, reps=10):
+    """OctreeSDF.sdf(x, lod) on 2^22 uniform points in [-1, 1]^3 (seed 2),
+    both forms, in input order and in Morton order (nglod_sdf_fwd_ordered,
+    the sort included in the timed call)."""
+    import torch
+
+    from nglod_amd import _lib
+
+    g = torch.Generator().manual_seed(2)
+    x = (torch.rand(1 << 22, 3, generator=g) * 2.0 - 1.0).to(dev)
+    out = {"points": int(x.shape[0]), "lod": lod}
+    for fname, form in (("fp32", _lib.FP32), ("mixed", _lib.MIXED)):
+        for ordered in (False, True):
+            for _ in range(2):
+                net.sdf_hip(x, lod, form=form, ordered=ordered)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                net.sdf_hip(x, lod, form=form, ordered=ordered)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            out[f"{fname}_{'morton' if ordered else 'input_order'}"] = {"ms": ms, "queries_per_s": x.shape[0] / (ms * 1e-3)}
+    return out
+
+
 def serve_bench(args, dev):
     """Mixed-request stream over 8 resident models (seeds 0-7: displaced
     icospheres, each fitted args.serve_fit_iters Adam steps): saturation
@@ -165,6 +192,7 @@ def main():
     ap.add_argument("--model-cache", default="", help="load the fitted model from / save it to this file")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 5 copies of 1 GiB (the counter calibration for HBM traffic) and exit")
+    ap.add_argument("--queries", type=int, default=1, help="also time sdf(x, lod) on 2^22 uniform points")
     ap.add_argument("--serve-requests", type=int, default=100,
                     help="requests per arrival rate in the serving run (0 = skip)")
     ap.add_argument("--serve-fit-iters", type=int, default=50, help="Adam steps per served model")
@@ -302,6 +330,8 @@ def main():
                       "achieved_GBs": r["achieved_GBs"]},
         },
     }
+    if args.queries:
+        line["config"]["queries"] = query_bench(net, dev, args.lod)
     if args.serve_requests > 0:
         line["serving"] = serve_bench(args, dev)
     if args.cpu_sample > 0:

@@ -119,6 +119,20 @@ int nglod_grid_sample(const float* grid, int32_t res, int32_t feature_dim, const
 int nglod_sdf_fwd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
                   int32_t form, float* out, void* stream);
 
+/* Workspace bytes nglod_sdf_fwd_ordered needs for n points. */
+size_t nglod_sdf_fwd_ordered_workspace_size(int64_t n);
+
+/* nglod_sdf_fwd (the same reference interface, OctreeSDF.sdf) for large,
+ * spatially scattered query sets: the points are first sorted by the Morton
+ * code of their position (10 bits per axis over [-1, 1]) and evaluated in that
+ * order, so the lanes of a wave gather neighbouring grid corners; out is in
+ * input order and bit-identical to nglod_sdf_fwd's (each point's arithmetic
+ * does not depend on the other points of its wave).  n < 2^31.  workspace:
+ * device memory of nglod_sdf_fwd_ordered_workspace_size(n) bytes, 16-byte
+ * aligned, owned by this call while it runs. */
+int nglod_sdf_fwd_ordered(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
+                          int32_t form, float* out, void* workspace, size_t workspace_bytes, void* stream);
+
 /* Every level's prediction, OctreeSDF.sdf(x, return_lst=True)[0..lod]
  * (OctreeSDF.py:148-153).  out is [lod+1][n] fp32.  FP32 form only. */
 int nglod_sdf_fwd_all(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,

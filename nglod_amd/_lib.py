@@ -108,6 +108,8 @@ SIGNATURES = {
     "nglod_grid_sample": (ctypes.c_int, [_p, _i32, _i32, _p, _i64, _p, _p]),
     "nglod_sdf_fwd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _p, _p]),
     "nglod_sdf_fwd_all": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _p, _p]),
+    "nglod_sdf_fwd_ordered_workspace_size": (_sz, [_i64]),
+    "nglod_sdf_fwd_ordered": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _p, _p, _sz, _p]),
     "nglod_trace_dense_workspace_size": (_sz, [_i64, _i32]),
     "nglod_trace_dense": (
         ctypes.c_int,

@@ -127,7 +127,11 @@ enum : int { kPassMarch = 0, kPassFp32 = 1, kPassNormals = 2, kPassOther = 3, kP
 
 struct TraceArgs;
 // Mixed-precision launchers (mixed.hip).
-int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out, hipStream_t s);
+// Batch queries; order (optional) is the evaluation order, a permutation of [0, n).
+int launch_sdf_fp32(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out,
+                    const uint32_t* order, hipStream_t s);
+int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out,
+                     const uint32_t* order, hipStream_t s);
 int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& timer);
 
 }  // namespace nglod

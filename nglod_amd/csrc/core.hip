@@ -81,20 +81,36 @@ __global__ __launch_bounds__(256) void grid_sample_kernel(const float* __restric
     for (int v = 0; v < kC / 4; ++v) o4[v] = make_float4(f[4 * v], f[4 * v + 1], f[4 * v + 2], f[4 * v + 3]);
 }
 
+// Batch query; with `order`, thread i evaluates point order[i] (a spatially
+// sorted permutation: neighbouring lanes gather neighbouring corners).
 template <bool POSINV>
 __global__ __launch_bounds__(256) void sdf_fp32_kernel(ModelArgs m, int lod, int dec, const float* __restrict__ x,
-                                                       int64_t n, float* __restrict__ out) {
+                                                       int64_t n, float* __restrict__ out,
+                                                       const uint32_t* __restrict__ order) {
     __shared__ Fp32Lds lds;
     stage_fp32_lds(m, dec, &lds);
     __syncthreads();
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < n;
     if (__ballot(valid) == 0) return;  // whole wave past the end (no barrier follows)
+    const int64_t i = valid ? (order ? (int64_t)order[t] : t) : 0;
     const float px = valid ? x[3 * i] : 0.0f;
     const float py = valid ? x[3 * i + 1] : 0.0f;
     const float pz = valid ? x[3 * i + 2] : 0.0f;
     const float d = sdf_fp32_wave<POSINV>(m, lod, &lds, m.b1[dec][0], lane_id(), valid, px, py, pz);
     if (valid) out[i] = d;
+}
+
+int launch_sdf_fp32(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out,
+                    const uint32_t* order, hipStream_t s) {
+    const int64_t blocks = (n + 255) / 256;
+    if (m.pos_inv)
+        hipLaunchKernelGGL(sdf_fp32_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, m, lod, dec, x, n, out,
+                           order);
+    else
+        hipLaunchKernelGGL(sdf_fp32_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, m, lod, dec, x, n, out,
+                           order);
+    return check_launch("sdf_fp32_kernel");
 }
 
 template <bool POSINV>
@@ -230,13 +246,8 @@ int nglod_sdf_fwd(const nglod_model_t* model, const float* x, int64_t n, int32_t
     if (n == 0) return NGLOD_OK;
     if (!x || !out) return fail(NGLOD_ERR_ARG, "NULL pointer");
     const int dec = decoder_of(model, lod);
-    if (form == NGLOD_MIXED) return launch_sdf_mixed(m, lod, dec, x, n, out, (hipStream_t)stream);
-    const int64_t blocks = (n + 255) / 256;
-    if (m.pos_inv)
-        hipLaunchKernelGGL(sdf_fp32_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, dec, x, n, out);
-    else
-        hipLaunchKernelGGL(sdf_fp32_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, dec, x, n, out);
-    return check_launch("sdf_fp32_kernel");
+    if (form == NGLOD_MIXED) return launch_sdf_mixed(m, lod, dec, x, n, out, nullptr, (hipStream_t)stream);
+    return launch_sdf_fp32(m, lod, dec, x, n, out, nullptr, (hipStream_t)stream);
 }
 
 int nglod_sdf_fwd_all(const nglod_model_t* model, const float* x, int64_t n, int32_t lod, float* out, void* stream) {

@@ -61,14 +61,16 @@ __global__ void pack_w0_kernel(const float* __restrict__ w0, int din, int off, u
 // ---------------------------------------------------------------------------
 template <bool POSINV>
 __global__ __launch_bounds__(256) void sdf_mixed_kernel(ModelArgs m, int lod, int dec, const float* __restrict__ x,
-                                                        int64_t n, float* __restrict__ out) {
+                                                        int64_t n, float* __restrict__ out,
+                                                        const uint32_t* __restrict__ order) {
     __shared__ MixedLds lds;
     stage_mixed_lds(m, dec, &lds);
     __syncthreads();
     const uint32_t lane = lane_id();
     const float b1 = m.b1[dec][0];
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = t < n;
+    const int64_t i = valid ? (order ? (int64_t)order[t] : t) : 0;
     const float px = valid ? x[3 * i] : 0.0f;
     const float py = valid ? x[3 * i + 1] : 0.0f;
     const float pz = valid ? x[3 * i + 2] : 0.0f;
@@ -304,12 +306,15 @@ static int grid_for(K kernel, int64_t n, int block) {
     return (int)(want < cap ? (want < 1 ? 1 : want) : cap);
 }
 
-int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out, hipStream_t s) {
+int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out,
+                     const uint32_t* order, hipStream_t s) {
     const int64_t blocks = (n + 255) / 256;
     if (m.pos_inv)
-        hipLaunchKernelGGL(sdf_mixed_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, m, lod, dec, x, n, out);
+        hipLaunchKernelGGL(sdf_mixed_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, m, lod, dec, x, n, out,
+                           order);
     else
-        hipLaunchKernelGGL(sdf_mixed_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, m, lod, dec, x, n, out);
+        hipLaunchKernelGGL(sdf_mixed_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, m, lod, dec, x, n, out,
+                           order);
     return check_launch("sdf_mixed_kernel");
 }
 

@@ -109,6 +109,7 @@ class OctreeSDF(nn.Module):
         ])
         self.loss_preds = None
         self._mixed = None  # (key, grids_h, w0_bf16) cache of the mixed form
+        self._order_ws = {}  # device -> workspace of the spatially ordered batch query
 
     # -- reference API -----------------------------------------------------
     def encode(self, x):
@@ -202,12 +203,30 @@ class OctreeSDF(nn.Module):
         self._mixed = (key, grids_h, w0s)
         return grids_h, w0s
 
-    def sdf_hip(self, x, lod, form=_lib.FP32):
+    # Batches of at least this many points are evaluated in Morton order
+    # (nglod_sdf_fwd_ordered); the results are identical either way.
+    ORDERED_MIN = 1 << 16
+
+    def sdf_hip(self, x, lod, form=_lib.FP32, ordered=None):
+        """ordered: None picks by batch size; True / False forces the spatially
+        ordered / input-order evaluation."""
         xf = x.detach().reshape(-1, 3).float().contiguous()
-        out = torch.empty(xf.shape[0], 1, device=x.device, dtype=torch.float32)
+        n = xf.shape[0]
+        out = torch.empty(n, 1, device=x.device, dtype=torch.float32)
         m = self.descriptor(form)
         lib = _lib.load()
-        _lib.check(lib.nglod_sdf_fwd(m, _lib.ptr(xf), xf.shape[0], lod, form, _lib.ptr(out), _lib.stream_of(x)),
+        if ordered is None:
+            ordered = n >= self.ORDERED_MIN
+        if ordered and n > 0:
+            nbytes = int(lib.nglod_sdf_fwd_ordered_workspace_size(n))
+            ws = self._order_ws.get(x.device)
+            if ws is None or ws.numel() < nbytes:
+                ws = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
+                self._order_ws[x.device] = ws
+            _lib.check(lib.nglod_sdf_fwd_ordered(m, _lib.ptr(xf), n, lod, form, _lib.ptr(out), _lib.ptr(ws),
+                                                 ws.numel(), _lib.stream_of(x)), "nglod_sdf_fwd_ordered")
+            return out
+        _lib.check(lib.nglod_sdf_fwd(m, _lib.ptr(xf), n, lod, form, _lib.ptr(out), _lib.stream_of(x)),
                    "nglod_sdf_fwd")
         return out
 

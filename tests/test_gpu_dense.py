@@ -9,7 +9,7 @@ import pytest
 import torch
 import torch.nn.functional as F
 
-from nglod_amd import Renderer, RenderBuffer, SphereTracer, workloads
+from nglod_amd import Renderer, RenderBuffer, SphereTracer, _lib, workloads
 from oracle import oracle
 from tests.helpers import ambiguous_rays, arrays, rand_points, rand_rays, small_model
 
@@ -232,3 +232,16 @@ def test_renderer_render_lookat(dev):
     rb2 = r2.render_lookat(net, f=[0, 0, 3], t=[0, 0, 0], fov=30)
     assert torch.equal(rb.depth, rb2.depth) and torch.equal(rb.hit, rb2.hit)
     assert isinstance(rb2, RenderBuffer)
+
+
+@pytest.mark.parametrize("form", [_lib.FP32, _lib.MIXED])
+@pytest.mark.parametrize("n", [1, 65, 4099, 200003])
+def test_sdf_ordered_matches_input_order(dev, form, n):
+    """The Morton-ordered batch query returns, in input order, exactly what the
+    input-order query returns (points outside [-1, 1] included)."""
+    net = workloads.make_model(num_lods=5, seed=11, device=dev)
+    g = torch.Generator().manual_seed(n)
+    x = (torch.rand(n, 3, generator=g) * 2.4 - 1.2).to(dev)
+    a = net.sdf_hip(x, 4, form=form, ordered=False)
+    b = net.sdf_hip(x, 4, form=form, ordered=True)
+    assert torch.equal(a, b)
