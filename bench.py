@@ -147,6 +147,44 @@ def query_bench(net, dev, lod, reps=10):
     return out
 
 
+def train_bench(dev, reps=5):
+    """The training configuration: 2^19 samples per batch (1/3 uniform, 2/3
+    near the surface at sigma 0.01 and 0.003) of the seed-0 displaced level-4
+    icosphere (5120 triangles) with FP64 mesh2sdf targets, and one training
+    step (fused forward of every LOD, the hand-written deterministic backward,
+    Adam) of a fresh 5-LOD model on a fixed batch."""
+    import torch
+
+    from nglod_amd import mesh, trainer, workloads
+    from nglod_amd.meshdata import MeshData
+
+    V, F = mesh.displaced_icosphere(0)
+    md = MeshData(V, F, dev)
+    n = 1 << 19
+    pts, gts = md.training_batch(n, seed=7)
+    net = workloads.make_model(num_lods=5, pos_invariant=True, seed=0, device=dev)
+    tr = trainer.Trainer(net)
+    tr.step_geometry(pts, gts)
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    for k in range(reps):
+        md.sdf(pts)
+    e[1].record()
+    for k in range(reps):
+        md.training_batch(n, seed=8 + k)
+    e[2].record()
+    for k in range(reps):
+        tr.step_geometry(pts, gts)
+    e[3].record()
+    torch.cuda.synchronize()
+    m2s = e[0].elapsed_time(e[1]) / reps
+    return {"samples": n, "triangles": int(md.nf), "mesh2sdf_ms": m2s,
+            "mesh2sdf_pairs_per_s": n * md.nf / (m2s * 1e-3),
+            "batch_ms": e[1].elapsed_time(e[2]) / reps, "step_ms": e[2].elapsed_time(e[3]) / reps,
+            "loss": float(tr.last_loss.item())}
+
+
 def serve_bench(args, dev):
     """Mixed-request stream over 8 resident models (seeds 0-7: displaced
     icospheres, each fitted args.serve_fit_iters Adam steps): saturation
@@ -193,6 +231,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run 5 copies of 1 GiB (the counter calibration for HBM traffic) and exit")
     ap.add_argument("--queries", type=int, default=1, help="also time sdf(x, lod) on 2^22 uniform points")
+    ap.add_argument("--train", type=int, default=1, help="also time the 2^19-sample training batch and step")
     ap.add_argument("--serve-requests", type=int, default=100,
                     help="requests per arrival rate in the serving run (0 = skip)")
     ap.add_argument("--serve-fit-iters", type=int, default=50, help="Adam steps per served model")
@@ -332,6 +371,8 @@ def main():
     }
     if args.queries:
         line["config"]["queries"] = query_bench(net, dev, args.lod)
+    if args.train:
+        line["config"]["training"] = train_bench(dev)
     if args.serve_requests > 0:
         line["serving"] = serve_bench(args, dev)
     if args.cpu_sample > 0:

@@ -27,13 +27,14 @@
 
 namespace nglod {
 
-// Passes 1 and 2 read every level from global memory: with LDS copies of the
-// coarse levels (FP16 or FP32 layout) they measured 10-25% slower on the
-// 1280x720 LOD-4 frame.  Both run 256-thread blocks at the occupancy their
-// registers allow; capping registers for more waves per SIMD, or fewer,
-// measured slower.
+// Pass 1 reads every level from global memory: with LDS copies of the coarse
+// levels (FP16 or FP32 layout) it measured 10-20% slower on the 1280x720 LOD-4
+// frame.  Pass 2 copies LODs 0-1 to LDS like the FP32 march, in 512-thread
+// blocks (3-5% faster than 256-thread blocks without them; 768 or 1024
+// threads slower).  Capping registers for more waves per SIMD, or fewer,
+// measured slower for both.
 constexpr int kMixedBlock = 256;
-constexpr int kPass2Block = 256;
+constexpr int kPass2Block = 512;
 // Pass 1 claims rays 64 ids at a time (the FP32 march claims per idle lane):
 // measured faster for the mixed march, slightly slower for the FP32 one.
 constexpr int kPass1Chunk = 64;
@@ -200,7 +201,9 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
 template <bool POSINV, bool RETURNS>
 __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArgs a) {
     __shared__ Fp32Lds lds;
+    __shared__ CoarseLds32 coarse;
     stage_fp32_lds(a.m, a.dec, &lds);
+    stage_coarse32(a.m, a.coarse, &coarse);
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArg
         if (id >= 0) s.resume(a, a.handoff[id]);
         if (__ballot(s.phase != PH_IDLE) == 0) break;
         const bool busy = s.phase != PH_IDLE;
-        const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz, a.coarse, &coarse);
         if (busy) {
             ++nevals;
             s.consume(a, d);
