@@ -179,8 +179,9 @@ def train_bench(dev, reps=5):
     e[3].record()
     torch.cuda.synchronize()
     m2s = e[0].elapsed_time(e[1]) / reps
-    return {"samples": n, "triangles": int(md.nf), "mesh2sdf_ms": m2s,
-            "mesh2sdf_pairs_per_s": n * md.nf / (m2s * 1e-3),
+    pairs = n * md.nf / (m2s * 1e-3)
+    return {"samples": n, "triangles": int(md.nf), "mesh2sdf_ms": m2s, "mesh2sdf_pairs_per_s": pairs,
+            "mesh2sdf_fp64_TFLOPs_nominal": pairs * 200 / 1e12,  # ~200 FP64 FLOP per pair; peak 78.6
             "batch_ms": e[1].elapsed_time(e[2]) / reps, "step_ms": e[2].elapsed_time(e[3]) / reps,
             "loss": float(tr.last_loss.item())}
 
@@ -306,7 +307,13 @@ def main():
                 "trace_mixed_pass2_kernel": (pass_ms[1], n_fp32 * b32),
                 "normals_pass_kernel": (pass_ms[2], 6 * hits * b32 + hits * 13),
             }
+        # decoder FLOPs (2*H*D_in + 2*H per evaluation) over the frame's kernel
+        # time: the MFMA utilisation figure (FP32 MFMA peak 157.3, BF16 2500 TFLOP/s)
+        fl = 2 * 128 * 32 + 2 * 128
+        dec = {"fp32_TFLOPs": ev[0] * fl / (kernel_ms * 1e-3) / 1e12,
+               "bf16_TFLOPs": ev[1] * fl / (kernel_ms * 1e-3) / 1e12}
         return {
+            "decoder": dec,
             "rays_per_s": n * args.steps / wall,
             "ms_per_step": wall * 1e3 / args.steps,
             "kernel_ms": kernel_ms,
