@@ -6,8 +6,9 @@
 //      pre-activations Z = W0 e + b0; writes E (decoder input), Z and
 //      ds = W0^T dz (feature part), dz = g * W1 * [Z > 0].
 //   2. decoder gradients: each block reduces a fixed chunk of queries in
-//      query order into partial dW0, db0, dW1, db1; a second kernel sums the
-//      partials in chunk order (and over levels, in level order, for a
+//      query order into partial dW0 (FP32 MFMA), db0, dW1, db1; a second
+//      kernel sums fixed groups of partials in chunk order and a third the
+//      group sums in group order (and over levels, in level order, for a
 //      joint decoder).
 //   3. grid gradients, per level j: every (query, corner) contribution
 //      w_c * dg_j with dg_j = sum_{i >= j} ds_i gets the corner index as a
@@ -112,57 +113,92 @@ __global__ __launch_bounds__(256) void bwd_query_kernel(BwdArgs a) {
     }
 }
 
-// Partial decoder gradients of one level over one chunk of queries, in
-// query order.  Output layout per (level, chunk): dW0 [128][din], db0 [128],
-// dW1 [128], db1 [1].  Thread t owns hidden row j = t % 128 and half of the
-// input columns (t / 128).
+typedef float f32x16_bwd __attribute__((ext_vector_type(16)));
+
+// Partial decoder gradients of one level over one chunk of kChunk queries.
+// Output layout per (level, chunk): dW0 [128][din], db0 [128], dW1 [128],
+// db1 [1].  dW0 = dZ^T E over the chunk as FP32 MFMA 32x32x2 products
+// (v_mfma_f32_32x32x2_f32: lane l holds A[m = l%32][k = l/32] and
+// B[k = l/32][n = l%32]): wave w owns hidden rows 32w..32w+31, and each step
+// takes two queries, A[m][k] = dz of query 2s+k at row m, B[k][n] = E of that
+// query at column n.  The products are exact and accumulate in FP32 in query
+// order, so a partial is the same every run.  db0, dW1 and db1 accumulate in
+// the lanes beside it (each half-wave over its query of every pair, the two
+// halves added at the end).
 __global__ __launch_bounds__(256) void bwd_decoder_partial_kernel(BwdArgs a, int level, float* __restrict__ part,
                                                                   int64_t nchunks) {
     const int64_t chunk = blockIdx.x;
-    const int t = threadIdx.x;
-    const int j = t & (kH - 1);
-    const int half = t >> 7;
-    const int k0 = half ? (a.din + 1) / 2 : 0;
-    const int k1 = half ? a.din : (a.din + 1) / 2;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int col = lane & 31;
+    const int half = lane >> 5;
+    const int j = 32 * wave + col;
     const int dec = a.m.num_dec == 1 ? 0 : level;
     const float w1 = a.m.w1[dec][j];
-    float acc_w0[18];
-#pragma unroll
-    for (int k = 0; k < 18; ++k) acc_w0[k] = 0.0f;
-    float acc_b0 = 0.0f, acc_w1 = 0.0f, acc_b1 = 0.0f;
+    const bool two = a.din > 32;  // the xyz columns put E over 32 columns: a second tile
     const int64_t q0 = chunk * kChunk;
     const int64_t q1 = q0 + kChunk < a.n ? q0 + kChunk : a.n;
     const float* Zl = a.Z + (size_t)level * a.n * kH;
     const float* El = a.E + (size_t)level * a.n * a.din;
     const float* gl = a.g + (size_t)level * a.n;
-    for (int64_t q = q0; q < q1; ++q) {
-        const float gq = gl[q];
-        const float z = Zl[q * kH + j];
-        const float h = z > 0.0f ? z : 0.0f;
-        const float dz = z > 0.0f ? gq * w1 : 0.0f;
-        const float* Eq = El + q * a.din;
-#pragma unroll
-        for (int k = 0; k < 18; ++k)
-            if (k0 + k < k1) acc_w0[k] = __builtin_fmaf(dz, Eq[k0 + k], acc_w0[k]);
-        acc_b0 += dz;
-        acc_w1 = __builtin_fmaf(gq, h, acc_w1);
-        acc_b1 += gq;
+    f32x16_bwd c0 = {}, c1 = {};
+    float acc_b0 = 0.0f, acc_w1 = 0.0f, acc_b1 = 0.0f;
+    for (int64_t qb = q0; qb < q1; qb += 2) {
+        const int64_t q = qb + half;
+        float dz = 0.0f, e0 = 0.0f, e1 = 0.0f;
+        if (q < q1) {
+            const float gq = gl[q];
+            const float z = Zl[q * kH + j];
+            dz = z > 0.0f ? gq * w1 : 0.0f;
+            acc_b0 += dz;
+            acc_w1 = __builtin_fmaf(gq, z > 0.0f ? z : 0.0f, acc_w1);
+            acc_b1 += gq;
+            e0 = El[q * a.din + col];
+            if (two && 32 + col < a.din) e1 = El[q * a.din + 32 + col];
+        }
+        c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(dz, e0, c0, 0, 0, 0);
+        if (two) c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(dz, e1, c1, 0, 0, 0);
     }
     const int outs = kH * a.din + 2 * kH + 1;
     float* P = part + ((size_t)level * nchunks + chunk) * outs;
 #pragma unroll
-    for (int k = 0; k < 18; ++k)
-        if (k0 + k < k1) P[j * a.din + k0 + k] = acc_w0[k];
+    for (int i = 0; i < 16; ++i) {
+        const int row = 32 * wave + 8 * (i >> 2) + 4 * half + (i & 3);
+        P[row * a.din + col] = c0[i];
+        if (two && 32 + col < a.din) P[row * a.din + 32 + col] = c1[i];
+    }
+    acc_b0 += __shfl_xor(acc_b0, 32);
+    acc_w1 += __shfl_xor(acc_w1, 32);
+    acc_b1 += __shfl_xor(acc_b1, 32);
     if (half == 0) {
         P[kH * a.din + j] = acc_b0;
         P[kH * a.din + kH + j] = acc_w1;
-        if (j == 0) P[kH * a.din + 2 * kH] = acc_b1;
     }
+    if (threadIdx.x == 0) P[kH * a.din + 2 * kH] = acc_b1;
 }
 
-// Sums the partials of decoder `dec` over its levels (ascending) and chunks
+// First stage of the partials' sum: group g of level l (grid z) adds chunks
+// g*kGroup .. g*kGroup+kGroup-1 in order for every output.
+constexpr int kGroup = 32;
+
+__global__ void bwd_decoder_group_kernel(BwdArgs a, const float* __restrict__ part, int64_t nchunks,
+                                         float* __restrict__ gpart, int64_t ngroups) {
+    const int outs = kH * a.din + 2 * kH + 1;
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.z;
+    const int64_t g = blockIdx.y;
+    if (o >= outs || !((a.mask >> l) & 1)) return;
+    const int64_t c0 = g * kGroup;
+    const int64_t c1 = c0 + kGroup < nchunks ? c0 + kGroup : nchunks;
+    const float* P = part + (size_t)l * nchunks * outs + o;
+    float s = 0.0f;
+    for (int64_t c = c0; c < c1; ++c) s += P[c * outs];
+    gpart[((size_t)l * ngroups + g) * outs + o] = s;
+}
+
+// Sums the group sums of decoder `dec` over its levels (ascending) and groups
 // (ascending) and writes dW0, db0, dW1, db1.
-__global__ void bwd_decoder_final_kernel(BwdArgs a, int dec, const float* __restrict__ part, int64_t nchunks,
+__global__ void bwd_decoder_final_kernel(BwdArgs a, int dec, const float* __restrict__ gpart, int64_t ngroups,
                                          float* __restrict__ dw0, float* __restrict__ db0, float* __restrict__ dw1,
                                          float* __restrict__ db1) {
     const int outs = kH * a.din + 2 * kH + 1;
@@ -172,8 +208,8 @@ __global__ void bwd_decoder_final_kernel(BwdArgs a, int dec, const float* __rest
     for (int l = 0; l <= a.lod; ++l) {
         const int d = a.m.num_dec == 1 ? 0 : l;
         if (d != dec || !((a.mask >> l) & 1)) continue;
-        const float* P = part + (size_t)l * nchunks * outs + o;
-        for (int64_t c = 0; c < nchunks; ++c) s += P[c * outs];
+        const float* P = gpart + (size_t)l * ngroups * outs + o;
+        for (int64_t g = 0; g < ngroups; ++g) s += P[g * outs];
     }
     if (o < kH * a.din) dw0[o] = s;
     else if (o < kH * a.din + kH) db0[o - kH * a.din] = s;
@@ -287,10 +323,10 @@ __global__ void fill_f32_kernel(float* p, int64_t n) {
 }
 
 struct BwdLayout {
-    size_t E, Z, ds, part, keys_in, keys_out, vals_in, vals_out, sort_tmp, total;
+    size_t E, Z, ds, part, gpart, keys_in, keys_out, vals_in, vals_out, sort_tmp, total;
     size_t ukeys, counts, nruns, rstart, pstart, pcount, piece_part, rle_tmp;
     size_t sort_bytes, rle_bytes;
-    int64_t nchunks;
+    int64_t nchunks, ngroups;
     uint32_t max_runs, max_pieces;
 };
 
@@ -316,6 +352,9 @@ static int bwd_layout(int64_t n, int lod, int din, const int32_t* res, BwdLayout
     off = align256(off + levels * n * kC * sizeof(float));
     L->part = off;
     off = align256(off + levels * L->nchunks * outs * sizeof(float));
+    L->ngroups = (L->nchunks + kGroup - 1) / kGroup;
+    L->gpart = off;
+    off = align256(off + levels * L->ngroups * outs * sizeof(float));
     const size_t m = (size_t)n * 8;
     L->keys_in = off;
     off = align256(off + m * 4);
@@ -448,12 +487,17 @@ int nglod_sdf_bwd(const nglod_model_t* model, const float* x, int64_t n, int32_t
     rc = check_launch("bwd_decoder_partial_kernel");
     if (rc) return rc;
     const int outs = kH * din + 2 * kH + 1;
+    float* gpart = (float*)(ws + L.gpart);
+    hipLaunchKernelGGL(bwd_decoder_group_kernel, dim3((outs + 255) / 256, (unsigned)L.ngroups, (unsigned)(lod + 1)),
+                       dim3(256), 0, s, a, part, L.nchunks, gpart, L.ngroups);
+    rc = check_launch("bwd_decoder_group_kernel");
+    if (rc) return rc;
     for (int d = 0; d < m.num_dec; ++d) {
         bool used = false;
         for (int l = 0; l <= lod; ++l) used |= ((m.num_dec == 1 ? 0 : l) == d) && ((level_mask >> l) & 1);
         if (!used) continue;
-        hipLaunchKernelGGL(bwd_decoder_final_kernel, dim3((outs + 255) / 256), dim3(256), 0, s, a, d, part, L.nchunks,
-                           grads->w0[d], grads->b0[d], grads->w1[d], grads->b1[d]);
+        hipLaunchKernelGGL(bwd_decoder_final_kernel, dim3((outs + 255) / 256), dim3(256), 0, s, a, d, gpart,
+                           L.ngroups, grads->w0[d], grads->b0[d], grads->w1[d], grads->b1[d]);
     }
     rc = check_launch("bwd_decoder_final_kernel");
     if (rc) return rc;
