@@ -75,9 +75,13 @@ __global__ __launch_bounds__(256) void bwd_query_kernel(BwdArgs a) {
             E[0] = px;
             E[1] = py;
             E[2] = pz;
-        }
 #pragma unroll
-        for (int c = 0; c < kC; ++c) E[a.off + c] = f[c];
+            for (int c = 0; c < kC; ++c) E[a.off + c] = f[c];
+        } else {
+            float4* E4 = reinterpret_cast<float4*>(E);  // rows of 32 floats: 16-byte aligned
+#pragma unroll
+            for (int c = 0; c < kC / 4; ++c) E4[c] = make_float4(f[4 * c], f[4 * c + 1], f[4 * c + 2], f[4 * c + 3]);
+        }
         float* dsq = a.ds + ((size_t)l * a.n + q) * kC;
         if (!((a.mask >> l) & 1)) {
 #pragma unroll
@@ -92,24 +96,31 @@ __global__ __launch_bounds__(256) void bwd_query_kernel(BwdArgs a) {
         float de[kC];
 #pragma unroll
         for (int c = 0; c < kC; ++c) de[c] = 0.0f;
-        float* Zq = a.Z + ((size_t)l * a.n + q) * kH;
-        for (int j = 0; j < kH; ++j) {
-            float acc = 0.0f;
-            if (a.off) {
-                acc = __builtin_fmaf(ldc(W0, j * a.din + 0), px, acc);
-                acc = __builtin_fmaf(ldc(W0, j * a.din + 1), py, acc);
-                acc = __builtin_fmaf(ldc(W0, j * a.din + 2), pz, acc);
+        float4* Zq = reinterpret_cast<float4*>(a.Z + ((size_t)l * a.n + q) * kH);
+        for (int j4 = 0; j4 < kH; j4 += 4) {
+            float zz[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j4 + u;
+                float acc = 0.0f;
+                if (a.off) {
+                    acc = __builtin_fmaf(ldc(W0, j * a.din + 0), px, acc);
+                    acc = __builtin_fmaf(ldc(W0, j * a.din + 1), py, acc);
+                    acc = __builtin_fmaf(ldc(W0, j * a.din + 2), pz, acc);
+                }
+#pragma unroll
+                for (int k = 0; k < kC; ++k) acc = __builtin_fmaf(ldc(W0, j * a.din + a.off + k), f[k], acc);
+                const float z = acc + ldc(B0, j);
+                zz[u] = z;
+                const float dz = z > 0.0f ? gq * ldc(W1, j) : 0.0f;
+#pragma unroll
+                for (int k = 0; k < kC; ++k) de[k] = __builtin_fmaf(dz, ldc(W0, j * a.din + a.off + k), de[k]);
             }
-#pragma unroll
-            for (int k = 0; k < kC; ++k) acc = __builtin_fmaf(ldc(W0, j * a.din + a.off + k), f[k], acc);
-            const float z = acc + ldc(B0, j);
-            Zq[j] = z;
-            const float dz = z > 0.0f ? gq * ldc(W1, j) : 0.0f;
-#pragma unroll
-            for (int k = 0; k < kC; ++k) de[k] = __builtin_fmaf(dz, ldc(W0, j * a.din + a.off + k), de[k]);
+            Zq[j4 / 4] = make_float4(zz[0], zz[1], zz[2], zz[3]);
         }
+        float4* ds4 = reinterpret_cast<float4*>(dsq);
 #pragma unroll
-        for (int c = 0; c < kC; ++c) dsq[c] = de[c];
+        for (int c = 0; c < kC / 4; ++c) ds4[c] = make_float4(de[4 * c], de[4 * c + 1], de[4 * c + 2], de[4 * c + 3]);
     }
 }
 
@@ -235,7 +246,28 @@ __global__ void bwd_corner_keys_kernel(BwdArgs a, int level, uint32_t* __restric
     }
 }
 
-// Contribution of sorted entry e: w_c(x_q) * dg_j(q), dg_j = sum_{l >= j} ds_l.
+// ds_l(q) -> dg_l(q) = sum_{i >= l} ds_i(q) in place, added from the finest
+// level down (((ds_lod + ds_lod-1) + ...) + ds_l): one thread per query and
+// group of 4 channels.
+__global__ void bwd_suffix_kernel(BwdArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n * (kC / 4)) return;
+    const int64_t q = t / (kC / 4);
+    const int c4 = (int)(t % (kC / 4));
+    float4 acc = reinterpret_cast<const float4*>(a.ds + ((size_t)a.lod * a.n + q) * kC)[c4];
+    for (int l = a.lod - 1; l >= 0; --l) {
+        float4* d = reinterpret_cast<float4*>(a.ds + ((size_t)l * a.n + q) * kC) + c4;
+        const float4 v = *d;
+        acc.x += v.x;
+        acc.y += v.y;
+        acc.z += v.z;
+        acc.w += v.w;
+        *d = acc;
+    }
+}
+
+// Contribution of sorted entry e: w_c(x_q) * dg_j(q), dg_j = sum_{l >= j} ds_l
+// (ds already holds these suffix sums, bwd_suffix_kernel).
 __device__ __forceinline__ void corner_contrib(const BwdArgs& a, int level, uint32_t v, float (&acc)[kC]) {
     const int r = a.m.res[level];
     const int64_t q = v >> 3;
@@ -247,22 +279,15 @@ __device__ __forceinline__ void corner_contrib(const BwdArgs& a, int level, uint
     const float wy = ((c >> 1) & 1) ? ay.w1 : ay.w0;
     const float wz = ((c >> 2) & 1) ? az.w1 : az.w0;
     const float w = (wx * wy) * wz;
-    float dg[kC];
+    const float4* d4 = reinterpret_cast<const float4*>(a.ds + ((size_t)level * a.n + q) * kC);
 #pragma unroll
-    for (int k = 0; k < kC; ++k) dg[k] = 0.0f;
-    for (int l = a.lod; l >= level; --l) {
-        const float4* d4 = reinterpret_cast<const float4*>(a.ds + ((size_t)l * a.n + q) * kC);
-#pragma unroll
-        for (int k = 0; k < kC / 4; ++k) {
-            const float4 d = d4[k];
-            dg[4 * k] += d.x;
-            dg[4 * k + 1] += d.y;
-            dg[4 * k + 2] += d.z;
-            dg[4 * k + 3] += d.w;
-        }
+    for (int k = 0; k < kC / 4; ++k) {
+        const float4 d = d4[k];
+        acc[4 * k] = __builtin_fmaf(w, d.x, acc[4 * k]);
+        acc[4 * k + 1] = __builtin_fmaf(w, d.y, acc[4 * k + 1]);
+        acc[4 * k + 2] = __builtin_fmaf(w, d.z, acc[4 * k + 2]);
+        acc[4 * k + 3] = __builtin_fmaf(w, d.w, acc[4 * k + 3]);
     }
-#pragma unroll
-    for (int k = 0; k < kC; ++k) acc[k] = __builtin_fmaf(w, dg[k], acc[k]);
 }
 
 // Pieces: run r (a run of equal corner keys, start rstart[r], length cnt[r])
@@ -501,6 +526,11 @@ int nglod_sdf_bwd(const nglod_model_t* model, const float* x, int64_t n, int32_t
     }
     rc = check_launch("bwd_decoder_final_kernel");
     if (rc) return rc;
+    if (lod > 0) {
+        hipLaunchKernelGGL(bwd_suffix_kernel, dim3((unsigned)((n * (kC / 4) + 255) / 256)), dim3(256), 0, s, a);
+        rc = check_launch("bwd_suffix_kernel");
+        if (rc) return rc;
+    }
     // grid gradients: levels that receive gradient from some level i >= j
     uint32_t* kin = (uint32_t*)(ws + L.keys_in);
     uint32_t* kout = (uint32_t*)(ws + L.keys_out);
