@@ -376,6 +376,57 @@ int nglod_spc_codes_from_points(const float* x, const uint8_t* mask, int64_t n, 
                                 int64_t cap, int64_t* count, void* workspace, size_t workspace_bytes,
                                 size_t* workspace_needed, void* stream);
 
+/* ---- post-trace shading (Renderer's shadow, AO and matcap options) ------ */
+
+/* Ground plane and shadow rays (sdf-net/lib/renderer.py:131-160).  Per ray,
+ * plane_t = (ray_o.y - ground_y) / (-ray_d.y); the plane is hit when
+ * 0 < plane_t < 500 and plane_t < t.  A plane hit clears hit[i], sets
+ * t = plane_t, x = o + d * plane_t, normal = (0, 1, 0) (x, t, hit, normal are
+ * updated in place) and plane_hit[i] = 1.  Every ray gets a shadow ray:
+ * shadow_o = x + 0.1 * normal, shadow_d = normalize((noise + light) - shadow_o)
+ * (noise [n][3] may be NULL: zero; the reference draws it from N(0, 0.01)),
+ * light_hit = shadow_d . normal > 0.  light: host float[3] (the reference's
+ * light is (-1.5, 4.5, -1.5)).  The caller traces (shadow_o, shadow_d) with
+ * nglod_trace_dense; shadow = that trace's hit & light_hit. */
+int nglod_shadow_rays(const float* ray_o, const float* ray_d, int64_t n, float ground_y, const float* light,
+                      const float* noise, float* x, float* t, uint8_t* hit, float* normal, uint8_t* plane_hit,
+                      float* shadow_o, float* shadow_d, uint8_t* light_hit, void* stream);
+
+/* Workspace bytes of nglod_ao (one claim counter; reset by every call). */
+size_t nglod_ao_workspace_size(void);
+
+/* Ambient occlusion (sdf-net/lib/renderer.py:184-209): per ray with hit[i]
+ * (or plane_hit[i]; plane_hit may be NULL; the two sets are disjoint as
+ * nglod_shadow_rays leaves them), 40 SDF taps at x + normal * d_k,
+ * d_k = 0.025 * ((k + 1) / 41)^1.6; r = sdf (a plane hit: min(sdf, d_k));
+ * acc += 3.5 * relu(d_k - r - 0.0015); ao = clamp(1 - acc, 0.1, 1)^2; rays
+ * with neither flag get ao = 1.  The taps are FP32 in both forms (as the
+ * normals); ao [n]. */
+int nglod_ao(const nglod_model_t* model, const float* x, const float* normal, const uint8_t* hit,
+             const uint8_t* plane_hit, int64_t n, int32_t lod, int32_t form, float* ao, void* workspace,
+             size_t workspace_bytes, void* stream);
+
+/* Shadow map of a width x height frame (ray index = ix * height + iy)
+ * (sdf-net/lib/renderer.py:301-304): clamp(1 - (shadow_hit & light_hit) + 0.9,
+ * 0, 1), then scipy.ndimage.gaussian_filter(sigma = 2) over the [width][height]
+ * array (mode 'reflect', radius 8, FP64 accumulation, FP32 after each axis).
+ * map, tmp: float [width * height]. */
+int nglod_shadow_map(const uint8_t* shadow_hit, const uint8_t* light_hit, int32_t width, int32_t height, float* map,
+                     float* tmp, void* stream);
+
+/* Matcap shading (sdf-net/lib/renderer.py:279-307 with
+ * geoutils.spherical_envmap :253-275 and matcap_sampler :237-250).  view [n][3]
+ * (the rays' directions), mm: host float[3][3] (may be NULL: identity; the
+ * view is rotated by mm^T).  texture [tw][th][channels] float: the matcap
+ * image transposed to x-major, values 0..255, sampled bilinearly on
+ * linspace(0, 1, tw) x linspace(0, 1, th) in FP64, then / 255.  Pixels with
+ * hit[i] = 0 get rgb = 1 and normal = 1 (normal is updated in place).  rgb is
+ * then multiplied by shadow_map [n] and by ao [n] (each may be NULL).
+ * rgb [n][3]. */
+int nglod_matcap_shade(const float* view, float* normal, const uint8_t* hit, int64_t n, const float* mm,
+                       const float* texture, int32_t tw, int32_t th, int32_t channels, const float* shadow_map,
+                       const float* ao, float* rgb, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -145,6 +145,11 @@ SIGNATURES = {
                                           _sz, _p, _p]),
     "nglod_spc_codes_from_points": (ctypes.c_int, [_p, _p, _i64, _i32, _p, _i64, ctypes.POINTER(_i64), _p, _sz,
                                                    ctypes.POINTER(_sz), _p]),
+    "nglod_shadow_rays": (ctypes.c_int, [_p, _p, _i64, ctypes.c_float, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "nglod_ao_workspace_size": (_sz, []),
+    "nglod_ao": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _p, _p, _p, _i64, _i32, _i32, _p, _p, _sz, _p]),
+    "nglod_shadow_map": (ctypes.c_int, [_p, _p, _i32, _i32, _p, _p, _p]),
+    "nglod_matcap_shade": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i32, _i32, _i32, _p, _p, _p, _p]),
 }
 
 _lib = None

@@ -230,30 +230,6 @@ __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArg
     add_evals(a, 0, nevals, lane);
 }
 
-// Every lane claims the next id in [0, total) whose flag is set, from one
-// counter (one atomic per wave per round, as many ids as lanes still
-// looking); -1 once the ids run out (`drained`, wave-uniform).
-__device__ __forceinline__ int64_t claim_hits(uint32_t* counter, const uint8_t* flag, uint32_t total, uint32_t lane,
-                                              bool& drained) {
-    int64_t id = -1;
-    for (;;) {
-        const bool want = id < 0 && !drained;
-        const uint64_t mask = __ballot(want);
-        if (mask == 0) break;
-        const int cnt = __popcll(mask);
-        const int leader = __ffsll((unsigned long long)mask) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)cnt);
-        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-        if (want) {
-            const uint32_t idx = base + mask_rank(mask);
-            if (idx < total && flag[idx]) id = idx;
-        }
-        if (base + (uint32_t)cnt >= total) drained = true;
-    }
-    return id;
-}
-
 // Pass 3: FP32 central-difference normals of the hits (hit[ray] = 1, the hit
 // point in x[ray]), one hit per lane, six evaluations each; waves claim
 // consecutive rays, so their hits are neighbours in the frame.

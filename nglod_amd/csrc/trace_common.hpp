@@ -299,6 +299,30 @@ struct RayState {
     }
 };
 
+// Every lane claims the next id in [0, total) whose flag is set (flag2, when
+// not NULL: or whose flag2 is set), from one counter (one atomic per wave per round, as many ids as lanes still
+// looking); -1 once the ids run out (`drained`, wave-uniform).
+__device__ __forceinline__ int64_t claim_hits(uint32_t* counter, const uint8_t* flag, uint32_t total, uint32_t lane,
+                                              bool& drained, const uint8_t* flag2 = nullptr) {
+    int64_t id = -1;
+    for (;;) {
+        const bool want = id < 0 && !drained;
+        const uint64_t mask = __ballot(want);
+        if (mask == 0) break;
+        const int cnt = __popcll(mask);
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)cnt);
+        base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+        if (want) {
+            const uint32_t idx = base + mask_rank(mask);
+            if (idx < total && (flag[idx] || (flag2 && flag2[idx]))) id = idx;
+        }
+        if (base + (uint32_t)cnt >= total) drained = true;
+    }
+    return id;
+}
+
 // Hands ray ids to the lanes of one wave.  The frame is split into kQueues
 // contiguous ranges with one claim counter each (counters[0..kQueues-1]);
 // blockIdx % kQueues picks the wave's own range first (one per XCD), and a
