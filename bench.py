@@ -147,6 +147,62 @@ def query_bench(net, dev, lod, reps=10):
     return out
 
 
+def shade_bench(net, o, d, dev, lod, width, height, num_steps, reps=5):
+    """Renderer shading of the bench frame (renderer.py:131-209, :279-307):
+    ground-plane shadow rays, the shadow trace (mixed form), the blurred
+    shadow map, 40-tap FP32 ambient occlusion of every hit, matcap shading.
+    The plane sits at the frame's lowest hit point; the matcap texture is a
+    seeded 512 x 512 RGB array; shadow-ray jitter off."""
+    import numpy as np
+    import torch
+
+    from nglod_amd import SphereTracer
+    from nglod_amd.renderer import ambient_occlusion, matcap_shade, shadow_map, shadow_rays
+
+    tr = SphereTracer(num_steps=num_steps, form="mixed")
+    rb = tr(net, o, d)
+    torch.cuda.synchronize()
+    ground = float(rb.x[rb.hit][:, 1].min().item())
+    tex = torch.from_numpy(np.random.default_rng(0).integers(0, 256, size=(512, 512, 3)).astype(np.float32)).to(dev)
+
+    def timed(fn, k=reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(k):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / k
+
+    # the shadow pass updates x / depth / hit / normal in place: one copy per call
+    copies = [type(rb)(x=rb.x.clone(), depth=rb.depth.clone(), hit=rb.hit.clone(), normal=rb.normal.clone())
+              for _ in range(reps + 1)]
+    it = iter(copies)
+    ms_setup = timed(lambda: shadow_rays(o, d, next(it), ground))
+    plane, so, sd, light_hit = shadow_rays(o, d, rb, ground)
+    srb = tr(net, so, sd)
+    ms_strace = timed(lambda: tr(net, so, sd))
+    shadow = srb.hit & light_hit
+    ms_map = timed(lambda: shadow_map(shadow, shadow, width, height))
+    smap = shadow_map(shadow, shadow, width, height)
+    ms_ao = timed(lambda: ambient_occlusion(net, rb.x, rb.normal, rb.hit, plane, lod=lod))
+    ao = ambient_occlusion(net, rb.x, rb.normal, rb.hit, plane, lod=lod)
+    ms_matcap = timed(lambda: matcap_shade(d, rb.normal, rb.hit, tex, shadow=smap, ao=ao))
+    torch.cuda.synchronize()
+    n_hit, n_plane = int(rb.hit.sum().item()), int(plane.sum().item())
+    taps = 40 * (n_hit + n_plane)
+    return {
+        "ground_y": ground, "hits": n_hit, "plane_hits": n_plane, "shadowed": int(shadow.sum().item()),
+        "shadow_setup_ms": ms_setup, "shadow_trace_ms": ms_strace, "shadow_map_ms": ms_map,
+        "ao_ms": ms_ao, "ao_taps": taps, "ao_taps_per_s": taps / (ms_ao * 1e-3),
+        "ao_algorithmic_GBs": taps * bytes_per_query(lod, elem=4) / (ms_ao * 1e-3) / 1e9,
+        "matcap_ms": ms_matcap,
+        "total_ms": ms_setup + ms_strace + ms_map + ms_ao + ms_matcap,
+    }
+
+
 def train_bench(dev, reps=5):
     """The training configuration: 2^19 samples per batch (1/3 uniform, 2/3
     near the surface at sigma 0.01 and 0.003) of the seed-0 displaced level-4
@@ -233,6 +289,7 @@ def main():
                     help="only run 5 copies of 1 GiB (the counter calibration for HBM traffic) and exit")
     ap.add_argument("--queries", type=int, default=1, help="also time sdf(x, lod) on 2^22 uniform points")
     ap.add_argument("--train", type=int, default=1, help="also time the 2^19-sample training batch and step")
+    ap.add_argument("--shade", type=int, default=1, help="also time shadows, AO and matcap shading of the frame")
     ap.add_argument("--serve-requests", type=int, default=100,
                     help="requests per arrival rate in the serving run (0 = skip)")
     ap.add_argument("--serve-fit-iters", type=int, default=50, help="Adam steps per served model")
@@ -380,6 +437,8 @@ def main():
         line["config"]["queries"] = query_bench(net, dev, args.lod)
     if args.train:
         line["config"]["training"] = train_bench(dev)
+    if args.shade:
+        line["config"]["shading"] = shade_bench(net, o, d, dev, args.lod, args.width, args.height, args.num_steps)
     if args.serve_requests > 0:
         line["serving"] = serve_bench(args, dev)
     if args.cpu_sample > 0:
