@@ -156,7 +156,11 @@ size_t nglod_trace_dense_workspace_size(int64_t n, int32_t form);
  * evals_out (device, may be NULL): uint64[2], SDF evaluations in FP32 form
  * ([0]) and in mixed form ([1]), accumulated (the caller zeroes it).
  * ray_steps (device, may be NULL): per ray, the in-loop SDF evaluations
- * (the reference's loop trips in which the ray was active), int32 [n]. */
+ * (the reference's loop trips in which the ray was active), int32 [n].
+ * Environment: NGLOD_TRACE_SOLO_LANES (0..64, default 4): in the FP32
+ * passes, a wave with no rays left to claim and at most this many busy lanes
+ * evaluates their queries one at a time with the whole wave (same bits; 0
+ * turns it off). */
 int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const float* ray_d,
                       int64_t n, int32_t lod, const nglod_trace_params_t* params, int32_t form,
                       float* x, float* t, uint8_t* hit, float* normal,

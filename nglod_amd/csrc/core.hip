@@ -6,6 +6,7 @@
 //   nglod_normals_fd    central-difference normals     (diffutils.py:61-70)
 //   nglod_trace_dense   persistent sphere tracer       (SphereTracer.py:41-132)
 #include <stdio.h>
+#include <stdlib.h>
 
 #include "api_common.hpp"
 #include "fp32_wave.hpp"
@@ -168,23 +169,43 @@ template <bool POSINV>
 __global__ __launch_bounds__(kMarchBlock) void trace_dense_fp32_kernel(TraceArgs a) {
     __shared__ Fp32Lds lds;
     __shared__ CoarseLds32 coarse;
+    __shared__ __align__(16) float solo_sv[kMarchBlock / 64][kH];
+    __shared__ SoloLevels solo_lv;
     stage_fp32_lds(a.m, a.dec, &lds);
     stage_coarse32(a.m, a.coarse, &coarse);
+    stage_solo_levels(a.m, &solo_lv);
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
+    float* sv = solo_sv[threadIdx.x >> 6];
     RayFeed feed;
     feed.init(a.nrays);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
     uint32_t nevals = 0;
+    bool tail = false;
     for (;;) {
         feed.refill<0>(s, a, lane);
-        if (__ballot(s.phase != PH_IDLE) == 0) break;
+        const uint64_t act = __ballot(s.phase != PH_IDLE);
+        if (act == 0) break;
+        if (feed.empty() && __popcll(act) <= a.solo_lanes) {
+            tail = true;
+            break;
+        }
         const bool busy = s.phase != PH_IDLE;
         const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz, a.coarse, &coarse);
         if (busy) {
+            ++nevals;
+            s.consume(a, d);
+        }
+    }
+    // the wave's last few rays, one query at a time (fp32_solo_step)
+    while (tail) {
+        const uint64_t act = __ballot(s.phase != PH_IDLE);
+        if (act == 0) break;
+        const float d = fp32_solo_step<POSINV>(a, &lds, &coarse, &solo_lv, sv, b1, lane, act, s.qx, s.qy, s.qz);
+        if (s.phase != PH_IDLE) {
             ++nevals;
             s.consume(a, d);
         }
@@ -294,6 +315,14 @@ static constexpr int kCounterWords = 4 * kQueues;
 
 static size_t flag_bytes(int64_t n) { return ((size_t)n + 255) / 256 * 256; }
 
+// TraceArgs::solo_lanes: NGLOD_TRACE_SOLO_LANES (0 = off .. 64) or the default.
+static int solo_lanes_setting() {
+    const char* e = getenv("NGLOD_TRACE_SOLO_LANES");
+    if (!e || !*e) return kSoloLanesDefault;
+    const int v = atoi(e);
+    return v < 0 ? 0 : (v > 64 ? 64 : v);
+}
+
 size_t nglod_trace_dense_workspace_size(int64_t n, int32_t form) {
     if (n < 0) n = 0;
     return kWsCounters + (form == NGLOD_MIXED ? (size_t)n * 2 * sizeof(Handoff) + 2 * flag_bytes(n) : 0);
@@ -353,6 +382,7 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     // The FP32 form takes its normal taps in-lane: a separate normals pass
     // measured 3-4% slower for it.
     a.defer_normals = form == NGLOD_MIXED ? 1 : 0;
+    a.solo_lanes = solo_lanes_setting();
     a.coarse = coarse_plan(m, lod);
     a.replay_steps = params->refine_steps < 1 ? 1 : params->refine_steps;
     a.near_dist = (float)params->near_dist;

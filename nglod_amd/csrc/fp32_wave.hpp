@@ -162,6 +162,148 @@ __device__ __forceinline__ float decode_fp32_wave(const float (&f)[kC], const Fp
     return (lo ? part0 : part1) + b1;
 }
 
+// The SDF of ONE query (wave-uniform px, py, pz) computed by the whole wave,
+// bit-identical to sdf_fp32_wave's result for that query: for a wave with
+// one or two rays left, where a wave-wide step costs the latency of several
+// dependent rounds of gathers plus 64 MFMAs whatever the lane count.
+//   gather: lane c (and c + 32) owns channel c; every level's 8 corner values
+//     are loaded first (one round of memory latency), then summed with the
+//     same weights in the same level / corner order as gather_fp32(_coarse);
+//   layer 1: lane L computes hidden rows L and L + 64 as the FMA chain
+//     z = fma(W0[R][2kk+1], f[2kk+1], fma(W0[R][2kk], f[2kk], z)), kk = 0..15,
+//     which is how v_mfma_f32_32x32x2_f32 accumulates (exact products, one
+//     rounding per product, k = 0 first; checked bit for bit on the device);
+//   layer 2: the ReLU outputs go through `sv` (this wave's 128 floats of LDS)
+//     and lanes 0 / 32 sum rows in layer2_tile's order for half 0 / half 1.
+// The grid pointers and resolutions come from an LDS copy (SoloLevels): read
+// as vector values they add no scalar registers to the persistent kernels,
+// which already use nearly all of theirs.
+// Every lane of the wave must call this with the same arguments.
+struct SoloLevels {
+    const float* grid[NGLOD_MAX_LODS];
+    int res[NGLOD_MAX_LODS];
+};
+
+// Every thread of the block calls this before the block's __syncthreads().
+__device__ __forceinline__ void stage_solo_levels(const ModelArgs& m, SoloLevels* lv) {
+#pragma unroll
+    for (int l = 0; l < NGLOD_MAX_LODS; ++l) {
+        if ((int)threadIdx.x == l) {
+            lv->grid[l] = m.grid[l];
+            lv->res[l] = m.res[l];
+        }
+    }
+}
+
+template <bool POSINV>
+__device__ __forceinline__ float sdf_fp32_solo(const SoloLevels* lv, int lod, const Fp32Lds* s, float b1,
+                                               uint32_t lane, float* sv, float px, float py, float pz, CoarsePlan cp,
+                                               const CoarseLds32* cs) {
+    const int c = (int)(lane & 31);
+    float g[NGLOD_MAX_LODS][8];
+#pragma unroll
+    for (int l = 0; l < NGLOD_MAX_LODS; ++l) {
+        if (l > lod) break;
+        const int r = lv->res[l];
+        const int S = r + 1;
+        const AxisW ax = axis_weights(px, r);
+        const AxisW ay = axis_weights(py, r);
+        const AxisW az = axis_weights(pz, r);
+        if (l < cp.levels) {
+            const float* cf = reinterpret_cast<const float*>(cs->c);
+            const int i0 = (l == 0 ? 0 : cp.off1) + (az.i0 * S + ay.i0) * S + ax.i0;
+            const int sy = S, sz = S * S;
+            const int k4 = c >> 2, j = c & 3;
+            g[l][0] = cf[swz32(i0, k4) * 4 + j];
+            g[l][1] = cf[swz32(i0 + 1, k4) * 4 + j];
+            g[l][2] = cf[swz32(i0 + sy, k4) * 4 + j];
+            g[l][3] = cf[swz32(i0 + sy + 1, k4) * 4 + j];
+            g[l][4] = cf[swz32(i0 + sz, k4) * 4 + j];
+            g[l][5] = cf[swz32(i0 + sz + 1, k4) * 4 + j];
+            g[l][6] = cf[swz32(i0 + sz + sy, k4) * 4 + j];
+            g[l][7] = cf[swz32(i0 + sz + sy + 1, k4) * 4 + j];
+        } else {
+            const float* base = lv->grid[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC + c;
+            const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
+            g[l][0] = base[0];
+            g[l][1] = base[sx];
+            g[l][2] = base[sy];
+            g[l][3] = base[sy + sx];
+            g[l][4] = base[sz];
+            g[l][5] = base[sz + sx];
+            g[l][6] = base[sz + sy];
+            g[l][7] = base[sz + sy + sx];
+        }
+    }
+    float f = 0.0f;
+#pragma unroll
+    for (int l = 0; l < NGLOD_MAX_LODS; ++l) {
+        if (l > lod) break;
+        const int r = lv->res[l];
+        const AxisW ax = axis_weights(px, r);
+        const AxisW ay = axis_weights(py, r);
+        const AxisW az = axis_weights(pz, r);
+        const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
+        const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
+        f = __builtin_fmaf(g[l][0], wxy00 * az.w0, f);
+        f = __builtin_fmaf(g[l][1], wxy10 * az.w0, f);
+        f = __builtin_fmaf(g[l][2], wxy01 * az.w0, f);
+        f = __builtin_fmaf(g[l][3], wxy11 * az.w0, f);
+        f = __builtin_fmaf(g[l][4], wxy00 * az.w1, f);
+        f = __builtin_fmaf(g[l][5], wxy10 * az.w1, f);
+        f = __builtin_fmaf(g[l][6], wxy01 * az.w1, f);
+        f = __builtin_fmaf(g[l][7], wxy11 * az.w1, f);
+    }
+    // layer 1: rows R0 = lane (tiles 0, 1) and R1 = lane + 64 (tiles 2, 3)
+    const int m0 = (int)(lane & 31), h0 = (int)(lane >> 5), h1 = h0 + 2;
+    float z0 = 0.0f, z1 = 0.0f;
+#pragma unroll
+    for (int kq = 0; kq < 4; ++kq) {
+        const float4 a0 = s->afrag[(h0 * 4 + kq) * 64 + m0], a0k = s->afrag[(h0 * 4 + kq) * 64 + 32 + m0];
+        const float4 a1 = s->afrag[(h1 * 4 + kq) * 64 + m0], a1k = s->afrag[(h1 * 4 + kq) * 64 + 32 + m0];
+        const float ea0[4] = {a0.x, a0.y, a0.z, a0.w}, ea0k[4] = {a0k.x, a0k.y, a0k.z, a0k.w};
+        const float ea1[4] = {a1.x, a1.y, a1.z, a1.w}, ea1k[4] = {a1k.x, a1k.y, a1k.z, a1k.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int ch = 8 * kq + 2 * e;
+            const float f0 = __shfl(f, ch), f1 = __shfl(f, ch + 1);
+            z0 = __builtin_fmaf(ea0k[e], f1, __builtin_fmaf(ea0[e], f0, z0));
+            z1 = __builtin_fmaf(ea1k[e], f1, __builtin_fmaf(ea1[e], f0, z1));
+        }
+    }
+    auto relu_row = [&](float z, int R) {
+        float v = z + s->b0[R];
+        if (!POSINV)
+            v = __builtin_fmaf(s->wxyz[R][0], px, __builtin_fmaf(s->wxyz[R][1], py, __builtin_fmaf(s->wxyz[R][2], pz, v)));
+        return fmaxf(v, 0.0f);
+    };
+    sv[lane] = relu_row(z0, (int)lane);
+    sv[lane + 64] = relu_row(z1, (int)lane + 64);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // layer 2 in layer2_tile's order: lanes 0..31 sum half 0, lanes 32..63 half 1
+    const int half = (int)(lane >> 5);
+    float part = 0.0f;
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int gq = 0; gq < 4; ++gq) {
+            const int row0 = 32 * h + 8 * gq + 4 * half;
+            const float4 w = *reinterpret_cast<const float4*>(&s->w1[row0]);
+            const float4 v = *reinterpret_cast<const float4*>(&sv[row0]);
+            acc = __builtin_fmaf(w.x, v.x, acc);
+            acc = __builtin_fmaf(w.y, v.y, acc);
+            acc = __builtin_fmaf(w.z, v.z, acc);
+            acc = __builtin_fmaf(w.w, v.w, acc);
+        }
+        part += acc;
+    }
+    const float p0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(part), 0));
+    const float p1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(part), 32));
+    return (p0 + p1) + b1;
+}
+
 template <bool POSINV>
 __device__ __forceinline__ float sdf_fp32_wave(const ModelArgs& m, int lod, const Fp32Lds* s, float b1,
                                                uint32_t lane, bool active, float px, float py, float pz,

@@ -202,27 +202,49 @@ template <bool POSINV, bool RETURNS>
 __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArgs a) {
     __shared__ Fp32Lds lds;
     __shared__ CoarseLds32 coarse;
+    __shared__ __align__(16) float solo_sv[kPass2Block / 64][kH];
+    __shared__ SoloLevels solo_lv;
     stage_fp32_lds(a.m, a.dec, &lds);
     stage_coarse32(a.m, a.coarse, &coarse);
+    stage_solo_levels(a.m, &solo_lv);
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
+    float* sv = solo_sv[threadIdx.x >> 6];
     RayFeed feed;
     feed.init(a.nrays);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
     uint32_t nevals = 0;
+    bool tail = false;
     for (;;) {
         const int64_t id = feed.next_flagged<true>(a.handoff_queue, a.hflag, lane, s.phase == PH_IDLE);
         if (id >= 0) s.resume(a, a.handoff[id]);
-        if (__ballot(s.phase != PH_IDLE) == 0) break;
+        const uint64_t act = __ballot(s.phase != PH_IDLE);
+        if (act == 0) break;
+        if (feed.drained && feed.fmask == 0 && __popcll(act) <= a.solo_lanes) {
+            tail = true;
+            break;
+        }
         const bool busy = s.phase != PH_IDLE;
         const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz, a.coarse, &coarse);
         if (busy) {
             ++nevals;
             s.consume(a, d);
             // hand back only a ray that moves away from the surface (step growing)
+            if (RETURNS && s.phase == PH_MARCH && fabsf(s.D) > a.return_dist && fabsf(s.D) > fabsf(s.Dp))
+                s.hand_off_to(a.ret, a.rflag);
+        }
+    }
+    // the wave's last few rays, one query at a time (fp32_solo_step)
+    while (tail) {
+        const uint64_t act = __ballot(s.phase != PH_IDLE);
+        if (act == 0) break;
+        const float d = fp32_solo_step<POSINV>(a, &lds, &coarse, &solo_lv, sv, b1, lane, act, s.qx, s.qy, s.qz);
+        if (s.phase != PH_IDLE) {
+            ++nevals;
+            s.consume(a, d);
             if (RETURNS && s.phase == PH_MARCH && fabsf(s.D) > a.return_dist && fabsf(s.D) > fabsf(s.Dp))
                 s.hand_off_to(a.ret, a.rflag);
         }

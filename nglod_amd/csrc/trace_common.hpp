@@ -4,6 +4,7 @@
 
 #include "coarse_lds.hpp"
 #include "device_common.hpp"
+#include "fp32_wave.hpp"
 
 namespace nglod {
 
@@ -86,8 +87,13 @@ struct TraceArgs {
     uint32_t* normal_queue;   // [kQueues] claim counters of the normals pass
     float return_dist;        // FP32 step length above which a ray goes back to mixed
     int defer_normals;        // 1: a finished hit is written out without its normal (a later pass computes it)
+    int solo_lanes;           // FP32 passes: a wave with no rays left to claim and at most this many busy
+                              // lanes evaluates them one at a time with the whole wave (sdf_fp32_solo)
     int64_t nrays;  // the frame is split into kQueues ranges [nrays*q/kQueues, nrays*(q+1)/kQueues)
 };
+
+// Default of TraceArgs::solo_lanes; NGLOD_TRACE_SOLO_LANES (0..64) overrides it.
+constexpr int kSoloLanesDefault = 4;
 
 struct RayState {
     int64_t ray;
@@ -321,6 +327,31 @@ __device__ __forceinline__ int64_t claim_hits(uint32_t* counter, const uint8_t* 
         if (base + (uint32_t)cnt >= total) drained = true;
     }
     return id;
+}
+
+// The tail of a persistent FP32 pass: a wave with nothing left to claim and
+// at most TraceArgs::solo_lanes busy lanes leaves its main loop (its busy
+// count can only fall from there) and evaluates the remaining queries one at
+// a time with the whole wave (sdf_fp32_solo), the same bits as the wave-wide
+// path.  A separate loop after the main one, so that its registers do not
+// weigh on the main loop's (inlined into the main loop, the unused solo code
+// made the pass 25% slower).
+template <bool POSINV>
+__device__ __forceinline__ float fp32_solo_step(const TraceArgs& a, const Fp32Lds* lds, const CoarseLds32* coarse,
+                                                const SoloLevels* lv, float* sv, float b1, uint32_t lane,
+                                                uint64_t act, float qx, float qy, float qz) {
+    float d = 0.0f;
+    uint64_t mm = act;
+    while (mm) {
+        const int r = __ffsll((unsigned long long)mm) - 1;
+        mm &= mm - 1ull;
+        // broadcast through a lane shuffle: the address arithmetic of the
+        // solo gather then stays in vector registers
+        const float px = __shfl(qx, r), py = __shfl(qy, r), pz = __shfl(qz, r);
+        const float dr = sdf_fp32_solo<POSINV>(lv, a.lod, lds, b1, lane, sv, px, py, pz, a.coarse, coarse);
+        if ((int)lane == r) d = dr;
+    }
+    return d;
 }
 
 // Hands ray ids to the lanes of one wave.  The frame is split into kQueues

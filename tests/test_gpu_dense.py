@@ -245,3 +245,28 @@ def test_sdf_ordered_matches_input_order(dev, form, n):
     a = net.sdf_hip(x, 4, form=form, ordered=False)
     b = net.sdf_hip(x, 4, form=form, ordered=True)
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("form", ["fp32", "mixed"])
+@pytest.mark.parametrize("n", [700, 5000])
+def test_solo_step_same_bits(dev, form, n, monkeypatch):
+    """A wave with nothing left to claim and few busy lanes evaluates their
+    queries one at a time with the whole wave (FP32 passes); forced on for
+    every such step (NGLOD_TRACE_SOLO_LANES=64: with these small inputs the
+    ray feed empties at once, so nearly every FP32 evaluation takes that
+    path) the trace is bit-identical to the wave-wide MFMA path (=0)."""
+    for pos_invariant in (True, False):
+        net = workloads.make_sphere_model(radius=0.6, num_lods=5, pos_invariant=pos_invariant, device="cpu").to(dev)
+        net.lod = 4
+        o, d = rand_rays(n, 31 + n)
+        o[::7] *= 0.2  # some rays start inside the box
+        outs = []
+        for solo in ("0", "64"):
+            monkeypatch.setenv("NGLOD_TRACE_SOLO_LANES", solo)
+            steps = torch.zeros(n, dtype=torch.int32, device=dev)
+            rb = SphereTracer(num_steps=128, form=form)(net, gpu(o, dev), gpu(d, dev), ray_steps=steps)
+            torch.cuda.synchronize()
+            outs.append([t.cpu() for t in (rb.x, rb.depth, rb.hit, rb.normal, steps)])
+        for a, b in zip(*outs):
+            assert torch.equal(a, b)
+        assert outs[0][2].sum() > n // 10
