@@ -193,6 +193,20 @@ __device__ __forceinline__ float sdf_fp32(const ModelArgs& m, int lod, int dec, 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
 // Number of set bits of `mask` below this lane.
+// Position of the r-th (0-based) set bit of m; r < popcount(m).
+__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const int c = __popcll((m >> pos) & ((1ull << w) - 1ull));
+        if (r >= c) {
+            r -= c;
+            pos += w;
+        }
+    }
+    return pos;
+}
+
 __device__ __forceinline__ uint32_t mask_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));

@@ -155,10 +155,16 @@ __device__ __forceinline__ void add_evals(const TraceArgs& a, int slot, uint32_t
 // RESUME: a later round, over the rays pass 2 handed back (ret, rflag)
 // instead of the frame's rays.  A separate instantiation: compiled into the
 // first-round kernel, the resume path slowed its march by half even unused.
+// Up to this many busy lanes, a drained pass-1 wave gathers cooperatively
+// (gather_fp16_coop: 16 lanes per query).
+constexpr int kCoopLanes = 4;
+
 template <bool POSINV, bool RESUME>
 __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArgs a) {
     __shared__ MixedLds lds;
+    __shared__ MixedLevels levels;
     stage_mixed_lds(a.m, a.dec, &lds);
+    stage_mixed_levels(a.m, &levels);
     __syncthreads();
     const uint32_t lane = lane_id();
     const float b1 = a.m.b1[a.dec][0];
@@ -170,6 +176,8 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
     uint32_t nevals = 0;
     MarchHistory hist;
     hist.clear();
+    const int coop = a.solo_lanes < kCoopLanes ? a.solo_lanes : kCoopLanes;
+    bool tail = false;
     for (;;) {
         if (RESUME) {
             const int64_t id = feed.next_flagged<true>(a.ret_queue, a.rflag, lane, s.phase == PH_IDLE);
@@ -180,9 +188,35 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
         } else {
             feed.refill<kPass1Chunk>(s, a, lane);
         }
-        if (__ballot(s.phase != PH_IDLE) == 0) break;
+        const uint64_t act = __ballot(s.phase != PH_IDLE);
+        if (act == 0) break;
+        // nothing left to claim and few rays: the tail loop below
+        if ((RESUME ? feed.drained && feed.fmask == 0 : feed.empty()) && __popcll(act) <= coop) {
+            tail = true;
+            break;
+        }
         const bool busy = s.phase != PH_IDLE;
         const float d = sdf_mixed_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        if (busy) {
+            ++nevals;
+            hist.push(s);
+            s.consume(a, d, true);
+            if (s.phase == PH_REPLAY) {
+                hist.rewind(s, a);
+                s.hand_off_to(a.handoff, a.hflag);
+            }
+        }
+    }
+    // The wave's last few rays: the same steps with the gather spread over the
+    // idle lanes (same bits).  A separate loop, so that its registers do not
+    // weigh on the main loop's.
+    while (tail) {
+        const uint64_t act = __ballot(s.phase != PH_IDLE);
+        if (act == 0) break;
+        const bool busy = s.phase != PH_IDLE;
+        __half2 acc[kC / 2];
+        gather_fp16_coop(&levels, a.lod, lane, act, s.qx, s.qy, s.qz, acc);
+        const float d = decode_mixed_wave<POSINV>(acc, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
             hist.push(s);

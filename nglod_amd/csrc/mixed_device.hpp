@@ -132,16 +132,11 @@ __device__ __forceinline__ float layer2_partial(const f32x16& c, int h, uint32_t
 // Lanes without a query pass active = false: they load nothing, and a
 // 32-query tile with no active lane skips its MFMAs.  Each output column
 // depends only on its own query.
+// The decoder on the 64 lanes' FP16 feature sums acc (zero on inactive
+// lanes); wave-wide, as sdf_mixed_wave.
 template <bool POSINV>
-__device__ __forceinline__ float sdf_mixed_wave(const ModelArgs& m, int lod, const MixedLds* s, float b1,
-                                                uint32_t lane, bool active, float px, float py, float pz) {
-    __half2 acc[kC / 2];
-    if (active) {
-        gather_fp16(m, lod, px, py, pz, acc);
-    } else {
-#pragma unroll
-        for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
-    }
+__device__ __forceinline__ float decode_mixed_wave(const __half2 (&acc)[kC / 2], const MixedLds* s, float b1,
+                                                   uint32_t lane, bool active, float px, float py, float pz) {
     const uint64_t am = __ballot(active);
     const bool use0 = (uint32_t)am != 0u, use1 = (uint32_t)(am >> 32) != 0u;
     const uint32_t half = lane >> 5;
@@ -198,6 +193,106 @@ __device__ __forceinline__ float sdf_mixed_wave(const ModelArgs& m, int lod, con
     part0 += __shfl_xor(part0, 32);
     part1 += __shfl_xor(part1, 32);
     return (lo ? part0 : part1) + b1;
+}
+
+template <bool POSINV>
+__device__ __forceinline__ float sdf_mixed_wave(const ModelArgs& m, int lod, const MixedLds* s, float b1,
+                                                uint32_t lane, bool active, float px, float py, float pz) {
+    __half2 acc[kC / 2];
+    if (active) {
+        gather_fp16(m, lod, px, py, pz, acc);
+    } else {
+#pragma unroll
+        for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
+    }
+    return decode_mixed_wave<POSINV>(acc, s, b1, lane, active, px, py, pz);
+}
+
+// FP16 grid pointers and resolutions, copied to LDS so that the cooperative
+// gather reads them as vector values (the mixed march uses nearly all of its
+// scalar registers).
+struct MixedLevels {
+    const __half* grid_h[NGLOD_MAX_LODS];
+    int res[NGLOD_MAX_LODS];
+};
+
+// Every thread of the block calls this before the block's __syncthreads().
+__device__ __forceinline__ void stage_mixed_levels(const ModelArgs& m, MixedLevels* lv) {
+#pragma unroll
+    for (int l = 0; l < NGLOD_MAX_LODS; ++l) {
+        if ((int)threadIdx.x == l) {
+            lv->grid_h[l] = reinterpret_cast<const __half*>(m.grid_h[l]);
+            lv->res[l] = m.res[l];
+        }
+    }
+}
+
+// gather_fp16 for the busy lanes of a wave with at most 4 of them (act, a
+// wave-uniform mask), done by 16 lanes per query: lane 16 i + j computes
+// channel pair j of the i-th busy lane's query with every level's 8 corner
+// loads issued first (one round of memory latency instead of one per level),
+// then the same __hfma2 chain in the same level / corner order; each busy
+// lane then collects its 16 pairs.  Bit-identical to gather_fp16.  Every lane
+// of the wave must call this.
+__device__ __forceinline__ void gather_fp16_coop(const MixedLevels* lv, int lod, uint32_t lane, uint64_t act,
+                                                 float qx, float qy, float qz, __half2 (&acc)[kC / 2]) {
+    const int i = (int)(lane >> 4), j = (int)(lane & 15);
+    const int k = __popcll(act);
+    const bool helper = i < k;
+    const int src = helper ? nth_set_bit(act, i) : 0;
+    const float px = __shfl(qx, src), py = __shfl(qy, src), pz = __shfl(qz, src);
+    uint32_t g[NGLOD_MAX_LODS][8];
+    if (helper) {
+#pragma unroll
+        for (int l = 0; l < NGLOD_MAX_LODS; ++l) {
+            if (l > lod) break;
+            const int r = lv->res[l];
+            const int S = r + 1;
+            const AxisW ax = axis_weights(px, r);
+            const AxisW ay = axis_weights(py, r);
+            const AxisW az = axis_weights(pz, r);
+            const uint32_t* base = reinterpret_cast<const uint32_t*>(
+                lv->grid_h[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC) + j;
+            const size_t sx = kC / 2, sy = (size_t)S * kC / 2, sz = (size_t)S * S * kC / 2;
+            g[l][0] = base[0];
+            g[l][1] = base[sx];
+            g[l][2] = base[sy];
+            g[l][3] = base[sy + sx];
+            g[l][4] = base[sz];
+            g[l][5] = base[sz + sx];
+            g[l][6] = base[sz + sy];
+            g[l][7] = base[sz + sy + sx];
+        }
+    }
+    __half2 v = __float2half2_rn(0.0f);
+    if (helper) {
+#pragma unroll
+        for (int l = 0; l < NGLOD_MAX_LODS; ++l) {
+            if (l > lod) break;
+            const int r = lv->res[l];
+            const AxisW ax = axis_weights(px, r);
+            const AxisW ay = axis_weights(py, r);
+            const AxisW az = axis_weights(pz, r);
+            const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
+            const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
+            v = __hfma2(__builtin_bit_cast(__half2, g[l][0]), __float2half2_rn(wxy00 * az.w0), v);
+            v = __hfma2(__builtin_bit_cast(__half2, g[l][1]), __float2half2_rn(wxy10 * az.w0), v);
+            v = __hfma2(__builtin_bit_cast(__half2, g[l][2]), __float2half2_rn(wxy01 * az.w0), v);
+            v = __hfma2(__builtin_bit_cast(__half2, g[l][3]), __float2half2_rn(wxy11 * az.w0), v);
+            v = __hfma2(__builtin_bit_cast(__half2, g[l][4]), __float2half2_rn(wxy00 * az.w1), v);
+            v = __hfma2(__builtin_bit_cast(__half2, g[l][5]), __float2half2_rn(wxy10 * az.w1), v);
+            v = __hfma2(__builtin_bit_cast(__half2, g[l][6]), __float2half2_rn(wxy01 * az.w1), v);
+            v = __hfma2(__builtin_bit_cast(__half2, g[l][7]), __float2half2_rn(wxy11 * az.w1), v);
+        }
+    }
+    const bool mine = (act >> lane) & 1ull;
+    const int q = mine ? (int)mask_rank(act) : 0;
+    const uint32_t vb = __builtin_bit_cast(uint32_t, v);
+#pragma unroll
+    for (int c = 0; c < kC / 2; ++c) {
+        const uint32_t w = (uint32_t)__shfl((int)vb, 16 * q + c);
+        acc[c] = mine ? __builtin_bit_cast(__half2, w) : __float2half2_rn(0.0f);
+    }
 }
 
 }  // namespace nglod

@@ -361,19 +361,6 @@ __device__ __forceinline__ float fp32_solo_step(const TraceArgs& a, const Fp32Ld
 // ids per atomic (CHUNK = 0: as many as it has lanes asking) and hands them
 // out in lane order, so its rays stay neighbours in the frame.  With a flag
 // array, ids whose flag is 0 are skipped (the passes over ray-indexed records).
-// Position of the r-th (0-based) set bit of m; r < popcount(m).
-__device__ __forceinline__ int nth_set_bit(uint64_t m, int r) {
-    int pos = 0;
-#pragma unroll
-    for (int w = 32; w >= 1; w >>= 1) {
-        const int c = __popcll((m >> pos) & ((1ull << w) - 1ull));
-        if (r >= c) {
-            r -= c;
-            pos += w;
-        }
-    }
-    return pos;
-}
 
 struct RayFeed {
     int64_t nrays;
