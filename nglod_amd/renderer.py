@@ -101,7 +101,9 @@ _ao_ws = {}
 
 def ambient_occlusion(net, x, normal, hit, plane_hit=None, lod=None, form="fp32"):
     """Ambient occlusion (renderer.py:184-209): 40 SDF taps along the normal
-    of each hit (and plane hit), ao = clamp(1 - acc, 0.1, 1)^2; [N, 1]."""
+    of each hit (and plane hit), ao = clamp(1 - acc, 0.1, 1)^2; [N, 1].
+    The claim counter (workspace) is one per device: calls on different
+    streams of one device must not overlap."""
     _lib.require_gpu(x, normal)
     lod = lod if lod is not None else (net.lod if net.lod is not None else net.num_lods - 1)
     x = x.detach().float().reshape(-1, 3).contiguous()

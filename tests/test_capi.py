@@ -66,3 +66,35 @@ def test_host_argument_checks():
     p.num_steps = 8
     assert lib.nglod_trace_dense(ctypes.byref(m), 16, 16, 10, 4, ctypes.byref(p), 0, 16, 16, 16, 16,
                                  None, 0, None, None, None) == _lib.ERR_WORKSPACE
+
+
+def test_shading_argument_checks():
+    """The shading entry points reject bad arguments before any launch, and
+    accept empty frames."""
+    lib = _lib.load()
+    light = (ctypes.c_float * 3)(-1.5, 4.5, -1.5)
+    assert lib.nglod_shadow_rays(None, None, 0, 0.0, light, None, None, None, None, None, None, None, None, None,
+                                 None) == _lib.OK
+    assert lib.nglod_shadow_rays(None, None, -1, 0.0, light, None, None, None, None, None, None, None, None, None,
+                                 None) == _lib.ERR_ARG
+    assert lib.nglod_shadow_rays(None, None, 4, 0.0, light, None, None, None, None, None, None, None, None, None,
+                                 None) == _lib.ERR_ARG
+    assert lib.nglod_shadow_rays(16, 16, 4, 0.0, None, None, 16, 16, 16, 16, 16, 16, 16, 16, None) == _lib.ERR_ARG
+    assert lib.nglod_ao_workspace_size() >= 4
+    m = _lib.NglodModel()
+    m.num_lods, m.feature_dim, m.hidden_dim, m.num_decoders = 3, 32, 128, 1
+    for i in range(3):
+        m.res[i] = 4 << i
+        m.grid[i] = 4096 * (i + 1)
+    m.w0[0] = m.b0[0] = m.w1[0] = m.b1[0] = 4096
+    assert lib.nglod_ao(ctypes.byref(m), 16, 16, 16, None, 8, 2, 7, 16, 16, 256, None) == _lib.ERR_ARG  # form
+    assert lib.nglod_ao(ctypes.byref(m), 16, 16, 16, None, 8, 2, 0, 16, 16, 8, None) == _lib.ERR_WORKSPACE
+    assert lib.nglod_ao(ctypes.byref(m), None, 16, 16, None, 8, 2, 0, 16, 16, 256, None) == _lib.ERR_ARG
+    assert lib.nglod_ao(ctypes.byref(m), None, None, None, None, 0, 2, 0, None, None, 0, None) == _lib.OK
+    assert lib.nglod_shadow_map(None, None, 0, 7, None, None, None) == _lib.OK
+    assert lib.nglod_shadow_map(None, None, -1, 7, None, None, None) == _lib.ERR_ARG
+    assert lib.nglod_shadow_map(None, None, 3, 7, None, None, None) == _lib.ERR_ARG
+    assert lib.nglod_matcap_shade(16, 16, 16, 8, None, 16, 1, 8, 3, None, None, 16, None) == _lib.ERR_ARG  # tw < 2
+    assert lib.nglod_matcap_shade(16, 16, 16, 8, None, 16, 8, 8, 2, None, None, 16, None) == _lib.ERR_ARG  # 2 ch
+    assert lib.nglod_matcap_shade(None, 16, 16, 8, None, 16, 8, 8, 3, None, None, 16, None) == _lib.ERR_ARG
+    assert lib.nglod_matcap_shade(None, None, None, 0, None, None, 8, 8, 3, None, None, None, None) == _lib.OK
