@@ -25,6 +25,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "api_common.hpp"
+#include "fp32_wave.hpp"
 
 namespace nglod {
 
@@ -45,82 +46,173 @@ struct BwdArgs {
     float* ds;       // [lod+1][n][32]
 };
 
+// Per-block copy of one level's decoder for the two MFMA products of the
+// query kernel: fwd = W0 fragments for Z^T = W0 e (as the trace decoders,
+// Fp32Lds), bwd[h][g][j][lane] = W0[32h + 8g + 4 (lane / 32) + j][off + lane % 32]
+// for ds^T = W0^T dz, whose k-steps pair hidden rows 8g + j and 8g + 4 + j:
+// exactly the rows lanes n and n + 32 hold for query n in the first product's
+// output, so each lane feeds its own registers as the second product's B
+// operand.
+struct BwdLds {
+    Fp32Lds fwd;
+    float bwd[4 * 4 * 4 * 64];
+};
+
+__device__ __forceinline__ void stage_bwd_lds(const ModelArgs& m, int dec, int din, int off, BwdLds* s) {
+    stage_fp32_lds_raw(m.w0[dec], din, m.b0[dec], m.w1[dec], &s->fwd);
+    const float* W0 = m.w0[dec];
+    for (int t = threadIdx.x; t < 4 * 4 * 4 * 64; t += blockDim.x) {
+        const int lane = t & 63, j = (t >> 6) & 3, g = (t >> 8) & 3, h = t >> 10;
+        const int row = 32 * h + 8 * g + 4 * (lane >> 5) + j;
+        s->bwd[t] = W0[row * din + off + (lane & 31)];
+    }
+}
+
+// One wave per 64 queries, every level up to lod: the running trilinear
+// features (stored as E), then for a level with an incoming gradient
+// Z^T = W0 e + b0 as FP32 MFMA 32x32x2 tiles (stored as Z), dz = g * W1 * [Z > 0]
+// in the output registers, and ds^T = W0^T dz as a second chain of MFMA tiles
+// (stored as ds).  A 32-query tile with no query in range skips its MFMAs.
+// Every thread of the block runs every level (the decoder tables are
+// restaged between levels when each level has its own decoder).
 __global__ __launch_bounds__(256) void bwd_query_kernel(BwdArgs a) {
+    __shared__ BwdLds lds;
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.n) return;
-    const float px = a.x[3 * q], py = a.x[3 * q + 1], pz = a.x[3 * q + 2];
+    const bool valid = q < a.n;
+    const uint32_t lane = lane_id();
+    const uint32_t half = lane >> 5;
+    const bool lo = half == 0;
+    const uint64_t am = __ballot(valid);
+    const bool use0 = (uint32_t)am != 0u, use1 = (uint32_t)(am >> 32) != 0u;
+    const int64_t wave0 = q - (int64_t)lane;  // query of lane 0
+    const int64_t q0 = wave0 + (lane & 31), q1 = q0 + 32;  // this lane's column of tile 0 / tile 1
+    const float px = valid ? a.x[3 * q] : 0.0f, py = valid ? a.x[3 * q + 1] : 0.0f, pz = valid ? a.x[3 * q + 2] : 0.0f;
+    const float qx0 = __shfl(px, lane & 31), qy0 = __shfl(py, lane & 31), qz0 = __shfl(pz, lane & 31);
+    const float qx1 = __shfl(px, (lane & 31) + 32), qy1 = __shfl(py, (lane & 31) + 32),
+                qz1 = __shfl(pz, (lane & 31) + 32);
     float f[kC];
 #pragma unroll
     for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+    int staged = -1;
     for (int l = 0; l <= a.lod; ++l) {
-        const int r = a.m.res[l];
-        const int S = r + 1;
-        const AxisW ax = axis_weights(px, r);
-        const AxisW ay = axis_weights(py, r);
-        const AxisW az = axis_weights(pz, r);
-        const float* base = a.m.grid[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
-        const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
-        const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
-        const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
-        corner_fma(f, base, wxy00 * az.w0);
-        corner_fma(f, base + sx, wxy10 * az.w0);
-        corner_fma(f, base + sy, wxy01 * az.w0);
-        corner_fma(f, base + sy + sx, wxy11 * az.w0);
-        corner_fma(f, base + sz, wxy00 * az.w1);
-        corner_fma(f, base + sz + sx, wxy10 * az.w1);
-        corner_fma(f, base + sz + sy, wxy01 * az.w1);
-        corner_fma(f, base + sz + sy + sx, wxy11 * az.w1);
-        float* E = a.E + ((size_t)l * a.n + q) * a.din;
-        if (a.off) {
-            E[0] = px;
-            E[1] = py;
-            E[2] = pz;
+        if (valid) {
+            const int r = a.m.res[l];
+            const int S = r + 1;
+            const AxisW ax = axis_weights(px, r);
+            const AxisW ay = axis_weights(py, r);
+            const AxisW az = axis_weights(pz, r);
+            const float* base = a.m.grid[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
+            const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
+            const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
+            const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
+            corner_fma(f, base, wxy00 * az.w0);
+            corner_fma(f, base + sx, wxy10 * az.w0);
+            corner_fma(f, base + sy, wxy01 * az.w0);
+            corner_fma(f, base + sy + sx, wxy11 * az.w0);
+            corner_fma(f, base + sz, wxy00 * az.w1);
+            corner_fma(f, base + sz + sx, wxy10 * az.w1);
+            corner_fma(f, base + sz + sy, wxy01 * az.w1);
+            corner_fma(f, base + sz + sy + sx, wxy11 * az.w1);
+            float* E = a.E + ((size_t)l * a.n + q) * a.din;
+            if (a.off) {
+                E[0] = px;
+                E[1] = py;
+                E[2] = pz;
 #pragma unroll
-            for (int c = 0; c < kC; ++c) E[a.off + c] = f[c];
-        } else {
-            float4* E4 = reinterpret_cast<float4*>(E);  // rows of 32 floats: 16-byte aligned
+                for (int c = 0; c < kC; ++c) E[a.off + c] = f[c];
+            } else {
+                float4* E4 = reinterpret_cast<float4*>(E);  // rows of 32 floats: 16-byte aligned
 #pragma unroll
-            for (int c = 0; c < kC / 4; ++c) E4[c] = make_float4(f[4 * c], f[4 * c + 1], f[4 * c + 2], f[4 * c + 3]);
+                for (int c = 0; c < kC / 4; ++c) E4[c] = make_float4(f[4 * c], f[4 * c + 1], f[4 * c + 2], f[4 * c + 3]);
+            }
         }
-        float* dsq = a.ds + ((size_t)l * a.n + q) * kC;
         if (!((a.mask >> l) & 1)) {
+            if (valid) {
+                float4* ds4 = reinterpret_cast<float4*>(a.ds + ((size_t)l * a.n + q) * kC);
 #pragma unroll
-            for (int c = 0; c < kC; ++c) dsq[c] = 0.0f;
+                for (int c = 0; c < kC / 4; ++c) ds4[c] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
             continue;
         }
         const int dec = a.m.num_dec == 1 ? 0 : l;
-        const float* W0 = a.m.w0[dec];
-        const float* B0 = a.m.b0[dec];
-        const float* W1 = a.m.w1[dec];
-        const float gq = a.g[(size_t)l * a.n + q];
-        float de[kC];
-#pragma unroll
-        for (int c = 0; c < kC; ++c) de[c] = 0.0f;
-        float4* Zq = reinterpret_cast<float4*>(a.Z + ((size_t)l * a.n + q) * kH);
-        for (int j4 = 0; j4 < kH; j4 += 4) {
-            float zz[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int j = j4 + u;
-                float acc = 0.0f;
-                if (a.off) {
-                    acc = __builtin_fmaf(ldc(W0, j * a.din + 0), px, acc);
-                    acc = __builtin_fmaf(ldc(W0, j * a.din + 1), py, acc);
-                    acc = __builtin_fmaf(ldc(W0, j * a.din + 2), pz, acc);
-                }
-#pragma unroll
-                for (int k = 0; k < kC; ++k) acc = __builtin_fmaf(ldc(W0, j * a.din + a.off + k), f[k], acc);
-                const float z = acc + ldc(B0, j);
-                zz[u] = z;
-                const float dz = z > 0.0f ? gq * ldc(W1, j) : 0.0f;
-#pragma unroll
-                for (int k = 0; k < kC; ++k) de[k] = __builtin_fmaf(dz, ldc(W0, j * a.din + a.off + k), de[k]);
-            }
-            Zq[j4 / 4] = make_float4(zz[0], zz[1], zz[2], zz[3]);
+        if (dec != staged) {  // block-uniform
+            __syncthreads();
+            stage_bwd_lds(a.m, dec, a.din, a.off, &lds);
+            __syncthreads();
+            staged = dec;
         }
-        float4* ds4 = reinterpret_cast<float4*>(dsq);
+        // B operands of both query tiles (as decode_fp32_wave)
+        float B0[kC / 2], B1[kC / 2];
 #pragma unroll
-        for (int c = 0; c < kC / 4; ++c) ds4[c] = make_float4(de[4 * c], de[4 * c + 1], de[4 * c + 2], de[4 * c + 3]);
+        for (int kk = 0; kk < kC / 2; ++kk) {
+            const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(f[2 * kk]), __float_as_uint(f[2 * kk + 1]),
+                                                             false, false);
+            B0[kk] = __uint_as_float(sw[0]);
+            B1[kk] = __uint_as_float(sw[1]);
+        }
+        const float gq = valid ? a.g[(size_t)l * a.n + q] : 0.0f;
+        const float g0 = __shfl(gq, lane & 31), g1 = __shfl(gq, (lane & 31) + 32);
+        f32x16_t d0 = {}, d1 = {};  // ds^T tiles: channel x query
+#pragma unroll 1
+        for (int h = 0; h < 4; ++h) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                if (t == 0 ? !use0 : !use1) continue;
+                f32x16_t c = {};
+#pragma unroll
+                for (int kq = 0; kq < 4; ++kq) {
+                    const float4 w = lds.fwd.afrag[(h * 4 + kq) * 64 + lane];
+                    const float* B = t == 0 ? B0 : B1;
+                    c = __builtin_amdgcn_mfma_f32_32x32x2f32(w.x, B[4 * kq + 0], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x2f32(w.y, B[4 * kq + 1], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x2f32(w.z, B[4 * kq + 2], c, 0, 0, 0);
+                    c = __builtin_amdgcn_mfma_f32_32x32x2f32(w.w, B[4 * kq + 3], c, 0, 0, 0);
+                }
+                const int64_t qt = t == 0 ? q0 : q1;
+                const bool in = qt < a.n;
+                const float tx = t == 0 ? qx0 : qx1, ty = t == 0 ? qy0 : qy1, tz = t == 0 ? qz0 : qz1;
+                const float gt = t == 0 ? g0 : g1;
+                float* Zq = a.Z + ((size_t)l * a.n + (in ? qt : 0)) * kH;
+                f32x16_t dz;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int row0 = 32 * h + 8 * g + 4 * (int)half;
+                    float z[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        float v = c[4 * g + j] + lds.fwd.b0[row0 + j];
+                        if (a.off)
+                            v = __builtin_fmaf(lds.fwd.wxyz[row0 + j][0], tx,
+                                               __builtin_fmaf(lds.fwd.wxyz[row0 + j][1], ty,
+                                                              __builtin_fmaf(lds.fwd.wxyz[row0 + j][2], tz, v)));
+                        z[j] = v;
+                        dz[4 * g + j] = v > 0.0f ? gt * lds.fwd.w1[row0 + j] : 0.0f;
+                    }
+                    if (in) *reinterpret_cast<float4*>(Zq + row0) = make_float4(z[0], z[1], z[2], z[3]);
+                }
+                // ds^T += W0^T dz over this tile's 32 hidden rows (k-steps pair rows 8g + j, 8g + 4 + j)
+                f32x16_t& d = t == 0 ? d0 : d1;
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        d = __builtin_amdgcn_mfma_f32_32x32x2f32(lds.bwd[((h * 4 + g) * 4 + j) * 64 + lane],
+                                                                 dz[4 * g + j], d, 0, 0, 0);
+                }
+            }
+        }
+        // ds rows: lane n / n + 32 hold channels 8 i + 4 half + (0..3) of query n of each tile
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int64_t qt = t == 0 ? q0 : q1;
+            if (qt >= a.n) continue;
+            const f32x16_t& d = t == 0 ? d0 : d1;
+            float* dsq = a.ds + ((size_t)l * a.n + qt) * kC;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                *reinterpret_cast<float4*>(dsq + 8 * i + 4 * half) =
+                    make_float4(d[4 * i + 0], d[4 * i + 1], d[4 * i + 2], d[4 * i + 3]);
+        }
     }
 }
 
