@@ -1,7 +1,7 @@
 """CPU restatement of the Renderer's post-trace shading
 (sdf-net/lib/renderer.py:131-209 and :268-331, sdf-net/lib/geoutils.py:237-275).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/ and bench.py's cpu_baseline leg,
+TEST INFRASTRUCTURE ONLY: imported by tests/,
 as the checker.  The product package never imports it.
 
 FP32 steps are restated op by op in NumPy float32, in the reference's
