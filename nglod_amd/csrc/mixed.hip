@@ -90,6 +90,7 @@ struct MarchHistory {
     __device__ __forceinline__ void clear() {
         n = 0;
         init = false;
+        t0 = t1 = t2 = t3 = d0 = d1 = d2 = d3 = 0.0f;  // finite: rewind weighs every entry
     }
 
     __device__ __forceinline__ void push(const RayState& s) {
@@ -118,8 +119,14 @@ struct MarchHistory {
             if (j == 2 && j < jmax && fabsf(d1) < a.near_dist) ++j;
             if (j == 3 && j < jmax && fabsf(d2) < a.near_dist) ++j;
         }
-        const float tj = j == 1 ? t0 : j == 2 ? t1 : j == 3 ? t2 : t3;
-        const float dj = j == 1 ? d0 : j == 2 ? d1 : j == 3 ? d2 : d3;
+        // Entry j as a sum with weights 1 and 0 (exact: one product by 1, the
+        // rest add zeros; every entry is finite).  A ?: chain over the fields
+        // was compiled into an indexed load, which kept the whole history in
+        // scratch memory and stored it there on every step.
+        const float w1 = j == 1 ? 1.0f : 0.0f, w2 = j == 2 ? 1.0f : 0.0f;
+        const float w3 = j == 3 ? 1.0f : 0.0f, w4 = j == 4 ? 1.0f : 0.0f;
+        const float tj = ((t0 * w1 + t1 * w2) + t2 * w3) + t3 * w4;
+        const float dj = ((d0 * w1 + d1 * w2) + d2 * w3) + d3 * w4;
         if (init && j == n) {
             const BoxEntry e = box_entry(s.ox, s.oy, s.oz, s.dx, s.dy, s.dz);
             s.phase = PH_INIT;
