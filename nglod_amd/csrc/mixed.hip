@@ -94,12 +94,15 @@ struct MarchHistory {
     }
 
     __device__ __forceinline__ void push(const RayState& s) {
-        if (s.phase == PH_INIT) {
+        // The box-entry evaluation has no step yet (a fresh ray's D is unset);
+        // record 0 there: rewind weighs every entry, so all must be finite.
+        const bool entry = s.phase == PH_INIT;
+        if (entry) {
             n = 0;
             init = true;
         }
         t3 = t2; t2 = t1; t1 = t0; t0 = s.t;
-        d3 = d2; d2 = d1; d1 = d0; d0 = s.D;
+        d3 = d2; d2 = d1; d1 = d0; d0 = entry ? 0.0f : s.D;
         ++n;
     }
 
