@@ -19,6 +19,13 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, datasheet
+# Vector L1 -> VGPR return path of the CUs: 64 B/clk per CU (the rate
+# rocprof-compute's speed-of-light panel uses for "vL1D cache BW"), 256 CUs,
+# 2.4 GHz peak engine clock (MI355X datasheet): 39.3 TB/s.
+CUS, SCLK_GHZ, VL1D_BYTES_PER_CLK = 256, 2.4, 64
+VL1D_PEAK_GBS = VL1D_BYTES_PER_CLK * CUS * SCLK_GHZ
+BF16_MFMA_PEAK_TFLOPS = 2500.0
+FP32_MFMA_PEAK_TFLOPS = 157.3
 
 
 def cpu_model():
@@ -45,26 +52,159 @@ def load_traffic(name):
     return None
 
 
-def cpu_baseline(net, o, d, lod, num_steps, sample_rays, threads):
-    """The reference's PyTorch path restated (oracle/torch_ref.py) on a strided
-    sample of the frame's rays, on the host cores."""
+def host_cores():
+    """CPUs this process may run on (its affinity mask, which a container's
+    cpuset narrows), and os.cpu_count() for the record."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return usable, os.cpu_count() or usable
+
+
+def _median_time(fn, runs=5):
+    """1 warm-up call, then the median wall time of `runs` calls."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def cpu_baseline(net, o, d, lod, num_steps, sample_stride, threads, total_cores):
+    """The reference's PyTorch path restated (oracle/torch_ref.py: per-LOD
+    F.grid_sample, every level decoded twice, the SphereTracer loop with mask
+    compaction, FD normals) on the host cores, 1 warm-up + median of 5 for
+    each workload: W1 sdf(x, lod) on 2^16 uniform points (seed 1); W2 the
+    256x256 pinhole frame (eye (0,0,3), fov 30); and every
+    `sample_stride`-th ray of the benchmark frame.  `value` is the frame
+    sample's rays/s."""
     import torch
 
+    from nglod_amd import workloads
     from oracle.torch_ref import TorchOctreeSDF, sphere_trace
 
     torch.set_num_threads(threads)
     ref = TorchOctreeSDF.from_module(net)
     ref.lod = lod
-    n = o.shape[0]
-    stride = max(1, n // sample_rays)
-    oc = o[::stride].cpu().contiguous()
-    dc = d[::stride].cpu().contiguous()
-    t0 = time.perf_counter()
-    sphere_trace(ref, oc, dc, num_steps=num_steps)
-    dt = time.perf_counter() - t0
-    return {"value": oc.shape[0] / dt, "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"{oc.shape[0]} rays (every {stride}th ray of the same frame), torch {torch.__version__}, "
-                      f"{threads} threads, {cpu_model()}, {dt:.1f} s"}
+    x1 = workloads.uniform_points(1 << 16, seed=1, device="cpu")
+    with torch.no_grad():
+        w1 = _median_time(lambda: ref.sdf(x1, lod=lod))
+    o2, d2 = workloads.camera_rays(256, 256, device="cpu")
+    w2 = _median_time(lambda: sphere_trace(ref, o2, d2, num_steps=num_steps))
+    oc = o[::sample_stride].cpu().contiguous()
+    dc = d[::sample_stride].cpu().contiguous()
+    wf = _median_time(lambda: sphere_trace(ref, oc, dc, num_steps=num_steps))
+    return {"value": oc.shape[0] / wf, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{oc.shape[0]} rays (every {sample_stride}th ray of the benchmark frame), 1 warm-up + "
+                      f"median of 5, torch {torch.__version__}, {threads} threads on {threads} usable cores "
+                      f"(os.cpu_count() {total_cores}), {cpu_model()}",
+            "W1_sdf_2^16_points": {"s": w1, "points_per_s": x1.shape[0] / w1},
+            "W2_trace_256x256": {"s": w2, "rays_per_s": o2.shape[0] / w2},
+            "frame_sample": {"s": wf, "rays": int(oc.shape[0]), "rays_per_s": oc.shape[0] / wf}}
+
+
+def accuracy_vs_fp64(net, o, d, lod, num_steps, stride, forms):
+    """The accuracy half of the metric: both forms' frame outputs on every
+    `stride`-th ray against the FP64 CPU oracle (oracle.trace, prec f64):
+    hit agreement, mean and p99 hit-depth error over rays both call hits,
+    median and p99 normal angle."""
+    import numpy as np
+
+    from oracle import oracle
+
+    m = oracle.ModelArrays.from_module(net)
+    idx = np.arange(0, o.shape[0], stride)
+    ref = oracle.trace(m, o.cpu().numpy()[idx], d.cpu().numpy()[idx], lod, num_steps=num_steps, prec="f64")
+    out = {"sample_rays": int(idx.size), "stride": stride, "fp64_hits": int(ref["hit"].sum())}
+    for name, rb in forms.items():
+        hit = rb.hit.cpu().numpy()[idx]
+        both = hit & ref["hit"]
+        derr = np.abs(rb.depth[:, 0].cpu().numpy()[idx][both] - ref["t"][both])
+        a = rb.normal.cpu().numpy()[idx][both].astype(np.float64)
+        b = ref["normal"][both]
+        c = (a * b).sum(1) / np.maximum(np.linalg.norm(a, axis=1) * np.linalg.norm(b, axis=1), 1e-30)
+        ang = np.degrees(np.arccos(np.clip(c, -1.0, 1.0)))
+        out[name] = {"hit_agreement": float((hit == ref["hit"]).mean()), "depth_err_mean": float(derr.mean()),
+                     "depth_err_p99": float(np.percentile(derr, 99)), "depth_err_max": float(derr.max()),
+                     "normal_deg_median": float(np.median(ang)), "normal_deg_p99": float(np.percentile(ang, 99))}
+    return out
+
+
+def load_counters(name):
+    return load_traffic(name)
+
+
+def roofline(kname, kp, r, args, traffic_of, counters_of, ceiling):
+    """The dominant kernel against the resource that binds it.  Its
+    algorithmic bytes (device-counted SDF evaluations x bytes per query) are
+    what the lanes gather through the vector L1 (hit rate 96-98%), about 200x
+    its HBM traffic, so the bound is the vL1D return path, not HBM; the HBM
+    rate (copy-calibrated FETCH_SIZE + WRITE_SIZE) and the MFMA busy share
+    (SQ_VALU_MFMA_BUSY_CYCLES) are reported beside it from the committed
+    profiles."""
+    tr = traffic_of(f"{kname}_{args.width}x{args.height}_lod{args.lod}.traffic.json") or {}
+    ct = counters_of(f"{kname}_{args.width}x{args.height}_lod{args.lod}.counters.json") or {}
+    traffic = tr.get("bytes_per_frame")
+    hbm = None
+    if traffic:
+        gbs = traffic / (kp["ms"] * 1e-3) / 1e9
+        hbm = {"bytes_per_launch": traffic, "GBs": gbs, "peak": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS}
+    return {
+        "bound": "vl1d",
+        "achieved": kp["GBs"],
+        "peak": VL1D_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": kp["GBs"] / VL1D_PEAK_GBS,
+        "traffic": traffic,
+        "kernel": kname,
+        "kernel_ms": kp["ms"],
+        "algorithmic_bytes": kp["algorithmic_bytes"],
+        "bytes_per_query": {"fp32": bytes_per_query(args.lod, elem=4), "mixed": bytes_per_query(args.lod, elem=2)},
+        "peak_source": f"{VL1D_BYTES_PER_CLK} B/clk/CU x {CUS} CUs x {SCLK_GHZ} GHz",
+        "measured_vl1d_ceiling": ceiling,
+        "hbm": hbm,
+        "mfma_busy": ct.get("mfma_busy"),
+        "counters": ct.get("summary"),
+        "frame": {"kernel_ms": r["kernel_ms"], "algorithmic_bytes": r["algorithmic_bytes"],
+                  "achieved_GBs": r["achieved_GBs"]},
+    }
+
+
+def vl1d_ceiling(dev, reps=5):
+    """Measured vector-L1 ceiling: nglod_vl1d_probe (16 KB buffer, 8 blocks
+    of 256 lanes per CU, 4096 16-byte loads per lane), best of `reps` calls
+    timed with HIP events on the launch stream."""
+    import torch
+
+    from nglod_amd import _lib
+
+    buf = torch.arange(4096, dtype=torch.int32, device=dev)  # 16 KB
+    blocks, iters = CUS * 8, 4096
+    out = torch.empty(blocks * 256 * 4, dtype=torch.int32, device=dev)
+    lib = _lib.load()
+
+    def call():
+        _lib.check(lib.nglod_vl1d_probe(_lib.ptr(buf), buf.numel() * 4, iters, blocks, _lib.ptr(out),
+                                        out.numel() * 4, _lib.stream_of(buf)), "nglod_vl1d_probe")
+
+    call()
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        call()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    nbytes = blocks * 256 * iters * 16
+    gbs = nbytes / (best * 1e-3) / 1e9
+    return {"GBs": gbs, "bytes": nbytes, "ms": best,
+            "bytes_per_clk_per_cu_at_peak_clock": gbs * 1e9 / (CUS * SCLK_GHZ * 1e9)}
 
 
 def copy_probe(dev):
@@ -281,7 +421,8 @@ def main():
     ap.add_argument("--height", type=int, default=720)
     ap.add_argument("--lod", type=int, default=4)
     ap.add_argument("--num-steps", type=int, default=128)
-    ap.add_argument("--cpu-sample", type=int, default=65536, help="rays in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=14,
+                    help="CPU baseline on every n-th ray of the frame, and the FP64 accuracy sample (0 = skip)")
     ap.add_argument("--fit-iters", type=int, default=200, help="Adam steps of the seeded model fit")
     ap.add_argument("--sparse", type=int, default=1, help="also time the sparse (octree) path on the frame")
     ap.add_argument("--model-cache", default="", help="load the fitted model from / save it to this file")
@@ -369,7 +510,7 @@ def main():
         fl = 2 * 128 * 32 + 2 * 128
         dec = {"fp32_TFLOPs": ev[0] * fl / (kernel_ms * 1e-3) / 1e12,
                "bf16_TFLOPs": ev[1] * fl / (kernel_ms * 1e-3) / 1e12}
-        return {
+        return rb, {
             "decoder": dec,
             "rays_per_s": n * args.steps / wall,
             "ms_per_step": wall * 1e3 / args.steps,
@@ -385,7 +526,9 @@ def main():
         }
 
     forms = ["fp32", "mixed"] if args.form == "both" else [args.form]
-    res = {f: run_form(f) for f in forms}
+    res, frames = {}, {}
+    for f in forms:
+        frames[f], res[f] = run_form(f)
     sparse = sparse_frame(args, net, o, d, dev) if args.sparse else None
     primary = "mixed" if "mixed" in res else "fp32"
     r = res[primary]
@@ -418,20 +561,7 @@ def main():
             "sparse": sparse,
             "model_fit_s": t_fit,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": kp["GBs"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": kp["GBs"] / HBM_PEAK_GBS,
-            "traffic": (tr or {}).get("bytes_per_frame"),
-            "kernel": kname,
-            "kernel_ms": kp["ms"],
-            "algorithmic_bytes": kp["algorithmic_bytes"],
-            "bytes_per_query": {"fp32": bytes_per_query(args.lod, elem=4), "mixed": bytes_per_query(args.lod, elem=2)},
-            "frame": {"kernel_ms": r["kernel_ms"], "algorithmic_bytes": r["algorithmic_bytes"],
-                      "achieved_GBs": r["achieved_GBs"]},
-        },
+        "roofline": roofline(kname, kp, r, args, load_traffic, load_counters, vl1d_ceiling(dev)),
     }
     if args.queries:
         line["config"]["queries"] = query_bench(net, dev, args.lod)
@@ -442,8 +572,9 @@ def main():
     if args.serve_requests > 0:
         line["serving"] = serve_bench(args, dev)
     if args.cpu_sample > 0:
-        threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(net, o, d, args.lod, args.num_steps, args.cpu_sample, threads)
+        usable, total = host_cores()
+        line["cpu_baseline"] = cpu_baseline(net, o, d, args.lod, args.num_steps, args.cpu_sample, usable, total)
+        line["accuracy"] = accuracy_vs_fp64(net, o, d, args.lod, args.num_steps, 53, frames)
     print(json.dumps(line), flush=True)
 
 

@@ -431,6 +431,18 @@ int nglod_matcap_shade(const float* view, float* normal, const uint8_t* hit, int
                        const float* texture, int32_t tw, int32_t th, int32_t channels, const float* shadow_map,
                        const float* ao, float* rgb, void* stream);
 
+/* ---- calibration ------------------------------------------------------ */
+
+/* No counterpart in the reference: a measurement aid for the roofline of the
+ * gather kernels.  blocks x 256 lanes each read `iters` 16-byte words of buf
+ * (buf_bytes: a power of two in [1 KB, 1 MB], small enough to stay in each
+ * CU's vector L1), a wave instruction reading 1 KB of consecutive words, and
+ * write one 16-byte word each to out (out_bytes >= blocks * 4096).  Bytes
+ * read = blocks * 256 * iters * 16; timing the call gives the vector L1 ->
+ * register ceiling of 16-byte loads. */
+int nglod_vl1d_probe(const void* buf, int64_t buf_bytes, int32_t iters, int32_t blocks, void* out,
+                     int64_t out_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

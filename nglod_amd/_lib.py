@@ -150,6 +150,7 @@ SIGNATURES = {
     "nglod_ao": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _p, _p, _p, _i64, _i32, _i32, _p, _p, _sz, _p]),
     "nglod_shadow_map": (ctypes.c_int, [_p, _p, _i32, _i32, _p, _p, _p]),
     "nglod_matcap_shade": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i32, _i32, _i32, _p, _p, _p, _p]),
+    "nglod_vl1d_probe": (ctypes.c_int, [_p, _i64, _i32, _i32, _p, _i64, _p]),
 }
 
 _lib = None

@@ -26,35 +26,43 @@ LIGHT = (-1.5, 4.5, -1.5)
 
 
 def normalized_grid(width, height, device="cuda", jitter=False, generator=None):
-    """grid[x, y] -> normalized window coordinates (geoutils.py:140-154)."""
-    window_x = torch.linspace(-1, 1, steps=width, device=device) * (width / height)
-    window_y = torch.linspace(1, -1, steps=height, device=device)
+    """Window coordinates of a width x height image, [W, H, 2]: entry [i, j]
+    is (x_i, y_j) with x_i running over [-W/H, W/H] left to right and y_j over
+    [1, -1] top to bottom (geoutils.py:140-154), so flattening it gives ray
+    index i * H + j.  With jitter, x_i gains U[0,1)/W and y_j gains U[0,1)/H,
+    drawn from `generator` (the reference draws them unseeded)."""
+    xs = torch.linspace(-1, 1, steps=width, device=device) * (width / height)
+    ys = torch.linspace(1, -1, steps=height, device=device)
     if jitter:
-        window_x = window_x + torch.rand(width, device=device, generator=generator) * (1.0 / width)
-        window_y = window_y + torch.rand(height, device=device, generator=generator) * (1.0 / height)
-    return torch.stack(torch.meshgrid(window_x, window_y, indexing="ij")).permute(1, 2, 0)
+        xs = xs + torch.rand(width, device=device, generator=generator) * (1.0 / width)
+        ys = ys + torch.rand(height, device=device, generator=generator) * (1.0 / height)
+    return torch.stack((xs[:, None].expand(width, height), ys[None, :].expand(width, height)), dim=-1)
 
 
 def look_at(f, t, width, height, mode="persp", fov=90.0, device="cuda", jitter=False, generator=None):
-    """Ray origins and directions of a look-at camera (geoutils.py:180-206)."""
-    camera_origin = torch.tensor(f, dtype=torch.float32, device=device)
-    camera_view = F.normalize(torch.tensor(t, dtype=torch.float32, device=device) - camera_origin, dim=0)
-    camera_right = F.normalize(torch.cross(camera_view, torch.tensor([0.0, 1.0, 0.0], device=device), dim=0), dim=0)
-    camera_up = F.normalize(torch.cross(camera_right, camera_view, dim=0), dim=0)
-    coord = normalized_grid(width, height, device=device, jitter=jitter, generator=generator)
-    tanh = math.tan(math.radians(fov / 2))
-    ray_origin = camera_right * coord[..., 0, None] * tanh + camera_up * coord[..., 1, None] * tanh + \
-        camera_origin + camera_view
-    ray_origin = ray_origin.reshape(-1, 3)
-    ray_offset = camera_view.unsqueeze(0).repeat(ray_origin.shape[0], 1)
+    """Rays of a camera at `f` looking at `t` with world up +y
+    (geoutils.py:180-206), [W*H, 3] origins and unit directions.
+
+    The camera frame is forward = normalize(t - f), side =
+    normalize(forward x up_world), up = normalize(side x forward).  Pixel
+    (x, y) of normalized_grid sits on the image plane one unit ahead of the
+    eye at side*x*s + up*y*s + f + forward with s = tan(fov/2), summed in
+    that order (the order fixes the FP32 bits that nglod_camera_rays
+    reproduces).  'persp': every ray starts at f towards its plane point;
+    'ortho': rays start at the plane points and all run along forward."""
+    eye = torch.tensor(f, dtype=torch.float32, device=device)
+    forward = F.normalize(torch.tensor(t, dtype=torch.float32, device=device) - eye, dim=0)
+    world_up = torch.tensor([0.0, 1.0, 0.0], device=device)
+    side = F.normalize(torch.linalg.cross(forward, world_up), dim=0)
+    up = F.normalize(torch.linalg.cross(side, forward), dim=0)
+    win = normalized_grid(width, height, device=device, jitter=jitter, generator=generator).reshape(-1, 2)
+    s = math.tan(math.radians(fov / 2))
+    plane = side * win[:, 0:1] * s + up * win[:, 1:2] * s + eye + forward
+    if mode == "persp":
+        return eye.expand_as(plane).contiguous(), F.normalize(plane - eye, dim=-1)
     if mode == "ortho":
-        ray_dir = F.normalize(ray_offset, dim=-1)
-    elif mode == "persp":
-        ray_dir = F.normalize(ray_origin - camera_origin, dim=-1)
-        ray_origin = camera_origin.repeat(ray_dir.shape[0], 1)
-    else:
-        raise ValueError("Invalid camera mode!")
-    return ray_origin, ray_dir
+        return plane, F.normalize(forward.expand_as(plane), dim=-1)
+    raise ValueError(f"look_at: camera mode must be 'persp' or 'ortho', got {mode!r}")
 
 
 def _u8(t):

@@ -6,8 +6,9 @@ sdf-net/lib/tracer/{BaseTracer,SphereTracer,RenderBuffer}.py.
 persistent HIP kernel (nglod_trace_dense) instead of a Python loop with a
 host sync and a mask compaction per step.
 """
+import copy
 import ctypes
-from dataclasses import asdict, astuple, dataclass, fields
+from dataclasses import dataclass, fields
 from typing import List, Union
 
 import torch
@@ -25,7 +26,25 @@ def setparam(args, param, paramstr):
 
 @dataclass
 class RenderBuffer:
-    """Per-ray render outputs (sdf-net/lib/tracer/RenderBuffer.py:28-139)."""
+    """Per-ray render outputs; the field list and its order are the
+    reference's interface (sdf-net/lib/tracer/RenderBuffer.py:28-41), so
+    buffers unpack and concatenate like the reference's.
+
+    Contract of the methods (RenderBuffer.py:43-139):
+      * iterating yields the fields in declaration order;
+      * `a + b` joins two buffers field by field along dim 0; a field set in
+        only one of them is taken as is, a field set in neither stays None;
+      * cuda/cpu/detach/byte/float/numpy/reshape/transpose map one tensor
+        operation over every set field (transpose swaps the first two dims
+        of a [W, H, k] field);
+      * exrdict() is a copy of the set fields by name, with `rgb` renamed
+        `default`;
+      * image() is a preview buffer in the 0..255 range: hit and
+        relative_depth repeated to three channels, normal moved from [-1, 1]
+        to [0, 1], rgb as is, each times 255 (relative_depth goes to `depth`);
+      * mean(*buffers) sums each field over the buffers that set it and
+        divides by the number of buffers given.
+    """
 
     x: Union[torch.Tensor, None] = None
     min_x: Union[torch.Tensor, None] = None
@@ -41,74 +60,76 @@ class RenderBuffer:
     albedo: Union[torch.Tensor, None] = None
 
     def __iter__(self):
-        return iter(astuple(self))
+        return (getattr(self, f.name) for f in fields(self))
 
     def __add__(self, other):
-        def _proc(pair):
-            a, b = pair
-            if a is not None and b is not None:
-                return torch.cat(pair)
-            return a if a is not None else b
+        joined = {}
+        for f in fields(self):
+            mine, theirs = getattr(self, f.name), getattr(other, f.name)
+            if mine is None or theirs is None:
+                joined[f.name] = theirs if mine is None else mine
+            else:
+                joined[f.name] = torch.cat((mine, theirs))
+        return type(self)(**joined)
 
-        return self.__class__(*map(_proc, zip(self, other)))
-
-    def _apply(self, fn):
-        return self.__class__(**{f.name: (None if getattr(self, f.name) is None else fn(getattr(self, f.name)))
-                                 for f in fields(self)})
+    def _apply(self, op):
+        return type(self)(**{f.name: op(v) for f, v in zip(fields(self), self) if v is not None})
 
     def cuda(self):
-        return self._apply(lambda x: x.cuda())
+        return self._apply(torch.Tensor.cuda)
 
     def cpu(self):
-        return self._apply(lambda x: x.cpu())
+        return self._apply(torch.Tensor.cpu)
 
     def detach(self):
-        return self._apply(lambda x: x.detach())
+        return self._apply(torch.Tensor.detach)
 
     def byte(self):
-        return self._apply(lambda x: x.byte())
-
-    def reshape(self, *dims: List[int]):
-        return self._apply(lambda x: x.reshape(*dims))
-
-    def transpose(self):
-        return self._apply(lambda x: x.permute(1, 0, 2))
-
-    def numpy(self):
-        return self._apply(lambda x: x.numpy())
+        return self._apply(torch.Tensor.byte)
 
     def float(self):
-        return self._apply(lambda x: x.float())
+        return self._apply(torch.Tensor.float)
+
+    def numpy(self):
+        return self._apply(torch.Tensor.numpy)
+
+    def reshape(self, *dims: List[int]):
+        return self._apply(lambda v: v.reshape(*dims))
+
+    def transpose(self):
+        return self._apply(lambda v: v.permute(1, 0, 2))
 
     def exrdict(self):
-        d = {k: v for k, v in asdict(self).items() if v is not None}
-        if "rgb" in d:
-            d["default"] = d.pop("rgb")
-        return d
+        out = {f.name: copy.deepcopy(v) for f, v in zip(fields(self), self) if v is not None}
+        if "rgb" in out:
+            out["default"] = out.pop("rgb")
+        return out
 
     def image(self):
-        norm = lambda a: ((a + 1.0) / 2.0) if a is not None else None
-        bwrgb = lambda a: torch.cat([a] * 3, dim=-1) if a is not None else None
-        rgb8 = lambda a: (a * 255.0) if a is not None else None
-        return self.__class__(hit=rgb8(bwrgb(self.hit)), normal=rgb8(norm(self.normal)), rgb=rgb8(self.rgb),
-                              depth=rgb8(bwrgb(self.relative_depth)))
+        def scaled(v):
+            return None if v is None else v * 255.0
+
+        def grey3(v):
+            return None if v is None else torch.cat((v, v, v), dim=-1)
+
+        def unit(v):
+            return None if v is None else (v + 1.0) / 2.0
+
+        return type(self)(hit=scaled(grey3(self.hit)), depth=scaled(grey3(self.relative_depth)),
+                          normal=scaled(unit(self.normal)), rgb=scaled(self.rgb))
 
     @staticmethod
     def mean(*rblst):
-        rb = RenderBuffer()
-        n = len(rblst)
-        for _rb in rblst:
-            for f in fields(_rb):
-                attr, dest = getattr(_rb, f.name), getattr(rb, f.name)
-                if attr is not None and dest is not None:
-                    setattr(rb, f.name, dest + attr)
-                elif dest is None:
-                    setattr(rb, f.name, attr)
-        for f in fields(rb):
-            dest = getattr(rb, f.name)
-            if dest is not None:
-                setattr(rb, f.name, dest / float(n))
-        return rb
+        count = float(len(rblst))
+        out = {}
+        for f in fields(RenderBuffer):
+            total = None
+            for rb in rblst:
+                v = getattr(rb, f.name)
+                if v is not None:
+                    total = v if total is None else total + v
+            out[f.name] = None if total is None else total / count
+        return RenderBuffer(**out)
 
 
 class BaseTracer(object):

@@ -135,3 +135,46 @@ def test_hip_nuggets_golden(dev):
     ray, cell = _nugget_cells(spc.points.cpu().numpy(), spc.pyramid.numpy(), level, nug.cpu().numpy())
     np.testing.assert_array_equal(ray, g["ray"])
     np.testing.assert_array_equal(cell, g["cell"])
+
+
+@pytest.mark.gpu
+def test_dropin_sol_nglod_aabb_golden(dev):
+    """nglod_amd.sol_nglod.aabb has the reference extension's signature and
+    return shapes (sol_nglod_kernel.cu:161-188: [x [N,3], t [N,1], hit [N]
+    bool], x = origin where there is no hit) and matches the fixture and the
+    FP32 oracle bit for bit."""
+    from nglod_amd.sol_nglod import aabb
+    a = _load("aabb.npz")
+    o = torch.from_numpy(a["o"]).to(dev)
+    d = torch.from_numpy(a["d"]).to(dev)
+    out = aabb(o, d)
+    assert isinstance(out, list) and len(out) == 3
+    x, t, hit = out
+    assert x.shape == (o.shape[0], 3) and x.dtype == torch.float32
+    assert t.shape == (o.shape[0], 1) and t.dtype == torch.float32
+    assert hit.shape == (o.shape[0],) and hit.dtype == torch.bool
+    h = hit.cpu().numpy()
+    np.testing.assert_array_equal(h, a["hit"])
+    np.testing.assert_allclose(t[:, 0].cpu().numpy()[h], a["t"][h], rtol=0, atol=1e-6)
+    xo, to, ho = oracle.aabb(a["o"], a["d"], "f32")
+    np.testing.assert_array_equal(x.cpu().numpy(), xo)
+    np.testing.assert_array_equal(t[:, 0].cpu().numpy(), to)
+    np.testing.assert_array_equal(x.cpu().numpy()[~h], a["o"][~h])
+    with pytest.raises(TypeError):
+        aabb(o.double(), d.double())
+    e = aabb(o[:0], d[:0])
+    assert e[0].shape == (0, 3) and e[1].shape == (0, 1) and e[2].shape == (0,)
+
+
+@pytest.mark.gpu
+def test_dropin_mesh2sdf_gpu_golden(dev):
+    """nglod_amd.mesh2sdf.mesh2sdf_gpu(points, mesh) -> [dist [N] f32] as
+    mesh2sdf_kernel.cu:895-927 returns it, against the FP64 fixture."""
+    from nglod_amd import mesh2sdf
+    g = _load("mesh2sdf.npz")
+    out = mesh2sdf.mesh2sdf_gpu(torch.from_numpy(g["points"]).to(dev), torch.from_numpy(g["tris"]).to(dev))
+    assert isinstance(out, list) and len(out) == 1
+    dist = out[0]
+    assert dist.shape == (g["points"].shape[0],) and dist.dtype == torch.float32
+    np.testing.assert_allclose(dist.cpu().numpy(), g["dist"], rtol=0, atol=1e-6)
+    np.testing.assert_array_equal(np.sign(dist.cpu().numpy()), np.sign(g["dist"].astype(np.float32)))

@@ -87,3 +87,54 @@ def test_surface_sampler_area_weighting_and_noise():
     off = p2 - p
     np.testing.assert_allclose(off.std(0), 0.01, rtol=0.02)
     np.testing.assert_allclose(off.mean(0), 0.0, atol=2e-4)
+
+
+def _sphere_moments_ok(v, tol=5e-3):
+    """What uniformity on S^2 fixes (Archimedes' hat-box theorem): each
+    coordinate is uniform on [-1, 1], so E[c] = 0 and E[c^2] = 1/3; the
+    coordinates are uncorrelated; the azimuth is uniform on [0, 2 pi).
+    Returns the failed checks."""
+    v = v.astype(np.float64)
+    bad = []
+    if np.abs(np.linalg.norm(v, axis=1) - 1.0).max() > 1e-6:
+        bad.append("norm")
+    for k in range(3):
+        if abs(v[:, k].mean()) > tol:
+            bad.append(f"E[{k}]")
+        if abs((v[:, k] ** 2).mean() - 1.0 / 3.0) > tol:
+            bad.append(f"E[{k}^2]")
+        # flat histogram of each coordinate over [-1, 1] (20 bins)
+        h = np.histogram(v[:, k], bins=20, range=(-1.0, 1.0))[0] / v.shape[0]
+        if np.abs(h - 0.05).max() > 2e-3:
+            bad.append(f"hist[{k}]")
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        if abs((v[:, a] * v[:, b]).mean()) > tol:
+            bad.append(f"E[{a}{b}]")
+    phi = np.mod(np.arctan2(v[:, 1], v[:, 0]), 2.0 * np.pi)
+    h = np.histogram(phi, bins=24, range=(0.0, 2.0 * np.pi))[0] / v.shape[0]
+    if np.abs(h - 1.0 / 24.0).max() > 2e-3:
+        bad.append("hist[phi]")
+    return bad
+
+
+def test_direction_sampler_is_uniform_on_the_sphere():
+    """orc_sample_directions (sdf-net/lib/geoutils.py:31-40: z = 1 - 2u,
+    phi = 2 pi v) on 2^20 samples meets every moment and histogram that a
+    uniform law on S^2 fixes, and two plausible mistakes built from the same
+    uniform draws fail them: a hemisphere (z = 1 - u) and a half turn
+    (phi = pi v)."""
+    n = 1 << 20
+    assert _sphere_moments_ok(oracle.sample_directions(11, n)) == []
+    # the stream's draws 0 and 1, which the direction sampler consumes
+    u = (oracle.sample_uniform(11, n).astype(np.float64) + 1.0) * 0.5
+
+    def dirs(z, phi):
+        r = np.sqrt(np.maximum(1.0 - z * z, 0.0))
+        return np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)
+
+    good = dirs(1.0 - 2.0 * u[:, 0], 2.0 * np.pi * u[:, 1])
+    np.testing.assert_allclose(good, oracle.sample_directions(11, n), rtol=0, atol=1e-6)
+    hemisphere = _sphere_moments_ok(dirs(1.0 - u[:, 0], 2.0 * np.pi * u[:, 1]))
+    half_turn = _sphere_moments_ok(dirs(1.0 - 2.0 * u[:, 0], np.pi * u[:, 1]))
+    assert "E[2]" in hemisphere and "hist[2]" in hemisphere
+    assert "E[1]" in half_turn and "hist[phi]" in half_turn

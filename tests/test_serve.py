@@ -120,3 +120,50 @@ def test_poisson_stream_latencies(models):
     assert s["requests"] == 40 and s["rays"] == sum(r.rays for r in reqs)
     assert 0 < s["p50_ms"] <= s["p99_ms"]
     assert all(r.finish >= r.start >= r.arrival - 1e-6 for r in st.results)
+
+
+@pytest.fixture(scope="module")
+def fitted_models(dev):
+    """The serving workload's 8 resident models: seeds 0-7, each a 5-LOD
+    OctreeSDF fitted 50 Adam steps to its own displaced icosphere."""
+    from nglod_amd import trainer
+    return [trainer.fitted_icosphere_model(seed=s, device=dev, iterations=50)[0] for s in range(8)]
+
+
+@pytest.mark.gpu
+def test_serving_workload_full_size(fitted_models):
+    """The serving configuration at its own size: 8 fitted models, requests
+    over all five resolutions up to 1920x1080 and LODs 0-4 with orbit
+    cameras, arriving together so that they overlap on the device.  Every
+    request's outputs are bit-identical alone and in the mix, and a
+    1280x720 request's depths agree with the FP64 oracle on a strided
+    sample (hit agreement >= 0.99, mean hit-depth error <= 1e-3)."""
+    from nglod_amd.serve import RESOLUTIONS, orbit_eye
+    from oracle import oracle
+    from tests.helpers import arrays
+
+    rng = np.random.default_rng(7)
+    reqs = []
+    for i in range(10):
+        w, h = RESOLUTIONS[i % 5]
+        reqs.append(RenderRequest(model=i % 8, width=w, height=h, lod=(3 * i) % 5, eye=orbit_eye(rng), rid=i))
+    server = TraceServer(fitted_models, pool_bytes=2 << 30, slots=4, keep_outputs=True)
+    mix = server.run(reqs)
+    assert len(mix.results) == len(reqs) and mix.peak_inflight > 1
+    for req, res in zip(reqs, mix.results):
+        alone = server.run([RenderRequest(**{**req.__dict__, "arrival": 0.0})]).results[0]
+        for k in ("x", "depth", "hit", "normal"):
+            assert torch.equal(res.outputs[k], alone.outputs[k]), (req.rid, req.width, k)
+    big = [r for r in reqs if (r.width, r.height) == (1280, 720)][0]
+    out = mix.results[big.rid].outputs
+    assert out["hit"].sum().item() > 1000
+    idx = np.arange(0, big.rays, 53)
+    m = arrays(fitted_models[big.model])
+    ref = oracle.trace(m, out["ray_o"].cpu().numpy()[idx], out["ray_d"].cpu().numpy()[idx], big.lod,
+                       num_steps=128, prec="f64")
+    hit = out["hit"].cpu().numpy()[idx]
+    both = hit & ref["hit"]
+    assert (hit == ref["hit"]).mean() >= 0.99
+    assert both.sum() > 100
+    derr = np.abs(out["depth"][:, 0].cpu().numpy()[idx][both] - ref["t"][both])
+    assert derr.mean() <= 1e-3, derr.mean()
