@@ -237,27 +237,33 @@ def sparse_frame(args, net, o, d, dev):
     spc = SPC.from_mesh(mesh.triangles(V, F), 7, dev)
     sol = SolNGLOD.from_net(net, spc, fp16=True)
     target = args.lod + 2
+    # the frame path: single-pass traversal (no host read-back), then the march
+    bufs = spc.raytrace_async(o, d, target)
     for _ in range(args.warmup):
-        sol.trace(o, d, lod=args.lod)
+        spc.raytrace_async(o, d, target, out=bufs)
+        sol.trace(o, d, lod=args.lod, nuggets=bufs[:2])
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     t_nug = t_march = 0.0
     for _ in range(args.steps):
         ev[0].record()
-        nug = spc.raytrace(o, d, target)
+        spc.raytrace_async(o, d, target, out=bufs)
         ev[1].record()
-        rb = sol.trace(o, d, lod=args.lod, nuggets=nug)
+        rb = sol.trace(o, d, lod=args.lod, nuggets=bufs[:2])
         ev[2].record()
         torch.cuda.synchronize()
         t_nug += ev[0].elapsed_time(ev[1])
         t_march += ev[1].elapsed_time(ev[2])
+    status = bufs[2].cpu().tolist()
+    if status[1] != 0:
+        raise RuntimeError(f"sparse frame: traversal status {status} (capacity {bufs[0].shape[0]})")
     n = o.shape[0]
     ms = (t_nug + t_march) / args.steps
     return {"rays_per_s": n / (ms * 1e-3), "ms_per_frame": ms, "raytrace_ms": t_nug / args.steps,
-            "march_ms": t_march / args.steps, "nuggets": int(nug[0].shape[0]), "octree_level": 7,
+            "march_ms": t_march / args.steps, "nuggets": int(status[0]), "octree_level": 7,
             "leaves": int(spc.pyramid[0, 7]), "hits": int(rb.hit.sum().item()),
             "sdf_evals": int(sol.last_evals.item()),
-            "note": "raytrace_ms includes the host read-back of the nugget count (a setup-style synchronous call)"}
+            "traversal": "single pass (nglod_spc_raytrace_async), no host read-back"}
 
 
 def query_bench(net, dev, lod, reps=10):
@@ -387,7 +393,10 @@ def serve_bench(args, dev):
     icospheres, each fitted args.serve_fit_iters Adam steps): saturation
     throughput with every request at t = 0, then Poisson arrivals at 0.3, 0.6
     and 0.9 of the saturated request rate.  Resolutions uniform in {64^2,
-    256^2, 640x480, 1280x720, 1920x1080}, LOD uniform 0..4, orbit cameras."""
+    256^2, 640x480, 1280x720, 1920x1080}, LOD uniform 0..4, orbit cameras.
+    Both serving modes: pooled batches (one nglod_trace_pool per batch, up to
+    2 batches in flight on one stream) and one trace call per request on 4
+    streams; each mode's rates are relative to its own saturation."""
     import math
 
     import torch
@@ -397,16 +406,20 @@ def serve_bench(args, dev):
 
     models = [trainer.fitted_icosphere_model(seed=s, device=dev, iterations=args.serve_fit_iters)[0]
               for s in range(8)]
-    server = TraceServer(models, pool_bytes=2 << 30, slots=4, form="mixed")
-    server.run(request_stream(8, math.inf, 8, seed=99))  # warm-up
-    sat = server.run(request_stream(args.serve_requests, math.inf, 8, seed=100)).summary()
-    req_rate = sat["requests"] / sat["wall_s"]
-    out = {"models": 8, "slots": 4, "pool_bytes": server.pool.nbytes, "form": "mixed",
-           "saturation": {**sat, "requests_per_s": req_rate}, "rates": {}}
-    for k, frac in enumerate((0.3, 0.6, 0.9)):
-        st = server.run(request_stream(args.serve_requests, frac * req_rate, 8, seed=101 + k)).summary()
-        out["rates"][f"{frac:.1f}x"] = {**st, "arrival_rate_per_s": frac * req_rate}
-    del server, models
+    out = {"models": 8, "form": "mixed", "pool_bytes": 2 << 30}
+    for mode, slots in (("pooled", 2), ("streams", 4)):
+        server = TraceServer(models, pool_bytes=2 << 30, slots=slots, form="mixed", mode=mode)
+        server.run(request_stream(8, math.inf, 8, seed=99))  # warm-up
+        sat = server.run(request_stream(args.serve_requests, math.inf, 8, seed=100)).summary()
+        req_rate = sat["requests"] / sat["wall_s"]
+        res = {"slots": slots, "saturation": {**sat, "requests_per_s": req_rate}, "rates": {}}
+        for k, frac in enumerate((0.3, 0.6, 0.9)):
+            st = server.run(request_stream(args.serve_requests, frac * req_rate, 8, seed=101 + k)).summary()
+            res["rates"][f"{frac:.1f}x"] = {**st, "arrival_rate_per_s": frac * req_rate}
+        out[mode] = res
+        del server
+        torch.cuda.empty_cache()
+    del models
     torch.cuda.empty_cache()
     return out
 

@@ -167,6 +167,54 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
                       void* workspace, size_t workspace_bytes, uint64_t* evals_out,
                       int32_t* ray_steps, void* stream);
 
+/* ---- pooled trace (serving) -------------------------------------------- */
+
+/* No counterpart in the reference (its nearest analogue is the batch loop of
+ * Renderer.render, sdf-net/lib/renderer.py:118-123).  A device table of
+ * models for nglod_trace_pool: nglod_model_table_fill validates the models
+ * (packed for NGLOD_MIXED, all LODs) and copies their descriptors to `table`
+ * (device, nglod_model_table_bytes(n)).  Synchronous; refill after a model's
+ * buffers move. */
+size_t nglod_model_table_bytes(int32_t n);
+int nglod_model_table_fill(const nglod_model_t* models, int32_t n, void* table, size_t table_bytes, void* stream);
+
+/* One request of a pooled trace: its rays are pool slots [offset, offset+count)
+ * of the per-ray arrays; model indexes the model table. */
+typedef struct nglod_pool_request {
+    int32_t model;
+    int32_t lod;
+    int64_t offset;
+    int64_t count;
+} nglod_pool_request_t;
+
+/* Workspace bytes of nglod_trace_pool for a pool of pool_rays ray slots
+ * (hand-off records and flags indexed by slot: 49 B per slot).  Calls on
+ * disjoint slots may share it, also while they overlap. */
+size_t nglod_trace_pool_workspace_size(int64_t pool_rays);
+
+/* Device bytes of one nglod_trace_pool call's claim counters; calls that may
+ * overlap (on different streams) need their own. */
+#define NGLOD_POOL_COUNTER_BYTES 256
+
+/* nglod_trace_dense (NGLOD_MIXED, SphereTracer semantics) for up to 64
+ * requests at once, each with its own model (an index into model_table;
+ * `models` is the same host array, for validation) and LOD: one launch of the
+ * persistent mixed march over all their rays, one FP32 continuation pass and
+ * one normals pass.  ray_o, ray_d, x, t, hit, normal are the pool's per-slot
+ * arrays ([pool_rays][3] or [pool_rays]); each request's outputs land in its
+ * own slots and are bit-identical to a nglod_trace_dense call on those rays
+ * alone (the FP32 continuation pass here runs without the single-request
+ * kernels' one-query-at-a-time tail, which does not change results).  The
+ * requests' slot ranges must not overlap, their models must agree on
+ * pos_invariant, and params->return_dist must be 0.  workspace: the pool's
+ * records (nglod_trace_pool_workspace_size(pool_rays)); counters:
+ * NGLOD_POOL_COUNTER_BYTES of device memory for this call. */
+int nglod_trace_pool(const nglod_model_t* models, const void* model_table, int32_t num_models,
+                     const nglod_pool_request_t* requests, int32_t num_requests, const nglod_trace_params_t* params,
+                     const float* ray_o, const float* ray_d, float* x, float* t, uint8_t* hit, float* normal,
+                     int64_t pool_rays, void* workspace, size_t workspace_bytes, void* counters,
+                     uint64_t* evals_out, void* stream);
+
 /* Central-difference normals h = 1/192 then normalize(eps=1e-5)
  * (sdf-net/lib/diffutils.py:61-70, SphereTracer.py:128-130) at points x [n][3].
  * Computed in FP32 for either form. */
@@ -215,11 +263,34 @@ int nglod_spc_points(const uint8_t* octree, int64_t osize, int16_t* points, int6
  * index within level `target`), rays in ascending order, each ray's cells
  * front-to-back as the reference orders them.  Replaces spc_raytrace
  * (sol-renderer/include/spc/spc/spc_raytrace_cuda.cpp:141-199).  ray_offsets
- * (device, n+1): CSR offsets of each ray's nuggets.  Bit-identical list. */
+ * (device, n+1): CSR offsets of each ray's nuggets.  Bit-identical list.
+ * For target <= 9 this is nglod_spc_raytrace_async's single launch followed
+ * by one read-back of the count; deeper targets run a per-ray depth-first
+ * walk twice (count, then emit) with the count read back in between. */
 int nglod_spc_raytrace(const uint8_t* octree, const uint32_t* prefix, const int16_t* points,
                        const int32_t* pyramid, int32_t level, const float* ray_o, const float* ray_d, int64_t n,
                        int32_t target, int32_t* nuggets, int64_t cap, int64_t* count, uint32_t* ray_offsets,
                        void* workspace, size_t workspace_bytes, size_t* workspace_needed, void* stream);
+
+/* Workspace bytes nglod_spc_raytrace_async needs for n rays (a ticket
+ * counter and 8 bytes per tile of 64 rays). */
+size_t nglod_spc_raytrace_async_workspace_size(int64_t n);
+
+/* The same traversal and nugget list as nglod_spc_raytrace (spc_raytrace,
+ * sol-renderer/include/spc/spc/spc_raytrace_cuda.cpp:141-199) in one
+ * launch with no host read-back, for target <= 9 (else
+ * NGLOD_ERR_UNSUPPORTED).  Waves take tiles of 64 rays and walk the octree
+ * level by level in LDS with ballot/prefix-sum compaction; tiles find their
+ * output position by a decoupled look-back over the tiles' counts.  Writes
+ * ray_offsets [n+1] (complete even when the list is cut) and status (device,
+ * uint32[2]): [0] the nugget count, [1] flags: 1 = more than cap nuggets
+ * (only the first cap written; call again with a larger buffer), 2 = a tile's
+ * look-back gave up (an internal error; the outputs are not valid).
+ * nuggets may be NULL with cap = 0 (count and offsets only). */
+int nglod_spc_raytrace_async(const uint8_t* octree, const uint32_t* prefix, const int16_t* points,
+                             const int32_t* pyramid, int32_t level, const float* ray_o, const float* ray_d, int64_t n,
+                             int32_t target, int32_t* nuggets, int64_t cap, uint32_t* ray_offsets, uint32_t* status,
+                             void* workspace, size_t workspace_bytes, void* stream);
 
 /* info[i] = 1 where nugget i starts a new ray (d_MarkUniqueRays,
  * sol-renderer/sdfRenderer.cu:108-120). */

@@ -73,6 +73,12 @@ class NglodTraceParams(ctypes.Structure):
     ]
 
 
+class NglodPoolRequest(ctypes.Structure):
+    """Mirror of nglod_pool_request_t."""
+
+    _fields_ = [("model", _i32), ("lod", _i32), ("offset", _i64), ("count", _i64)]
+
+
 class NglodModelGrads(ctypes.Structure):
     """Mirror of nglod_model_grads_t."""
 
@@ -151,6 +157,15 @@ SIGNATURES = {
     "nglod_shadow_map": (ctypes.c_int, [_p, _p, _i32, _i32, _p, _p, _p]),
     "nglod_matcap_shade": (ctypes.c_int, [_p, _p, _p, _i64, _p, _p, _i32, _i32, _i32, _p, _p, _p, _p]),
     "nglod_vl1d_probe": (ctypes.c_int, [_p, _i64, _i32, _i32, _p, _i64, _p]),
+    "nglod_model_table_bytes": (_sz, [_i32]),
+    "nglod_model_table_fill": (ctypes.c_int, [ctypes.POINTER(NglodModel), _i32, _p, _sz, _p]),
+    "nglod_trace_pool_workspace_size": (_sz, [_i64]),
+    "nglod_trace_pool": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i32, ctypes.POINTER(NglodPoolRequest), _i32,
+                                        ctypes.POINTER(NglodTraceParams), _p, _p, _p, _p, _p, _p, _i64, _p, _sz, _p,
+                                        _p, _p]),
+    "nglod_spc_raytrace_async_workspace_size": (_sz, [_i64]),
+    "nglod_spc_raytrace_async": (ctypes.c_int, [_p, _p, _p, _p, _i32, _p, _p, _i64, _i32, _p, _i64, _p, _p, _p, _sz,
+                                                _p]),
 }
 
 _lib = None

@@ -134,4 +134,24 @@ int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64
                      const uint32_t* order, hipStream_t s);
 int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& timer);
 
+// Pooled trace (several requests, one launch per pass): the request table is
+// a kernel argument, the models a device table of ModelArgs.
+constexpr int kPoolMaxRequests = 64;
+struct PoolReq {
+    int32_t model;  // index into PoolArgs::models
+    int32_t lod;
+    int32_t dec;    // decoder of that lod
+    int32_t pad;
+    int64_t vbeg;   // first virtual id (requests in ascending vbeg)
+    int64_t off;    // first pool slot
+};
+struct PoolArgs {
+    const ModelArgs* models;
+    int64_t total;  // virtual ids of the launch
+    int32_t nreq;
+    int32_t pad;
+    PoolReq req[kPoolMaxRequests];
+};
+int launch_trace_pool(const TraceArgs& a, const PoolArgs& p, bool pos_inv, hipStream_t s, PassTimer& timer);
+
 }  // namespace nglod

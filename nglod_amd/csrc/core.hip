@@ -315,13 +315,6 @@ static constexpr int kCounterWords = 4 * kQueues;
 
 static size_t flag_bytes(int64_t n) { return ((size_t)n + 255) / 256 * 256; }
 
-// TraceArgs::solo_lanes: NGLOD_TRACE_SOLO_LANES (0 = off .. 64) or the default.
-static int solo_lanes_setting() {
-    const char* e = getenv("NGLOD_TRACE_SOLO_LANES");
-    if (!e || !*e) return kSoloLanesDefault;
-    const int v = atoi(e);
-    return v < 0 ? 0 : (v > 64 ? 64 : v);
-}
 
 size_t nglod_trace_dense_workspace_size(int64_t n, int32_t form) {
     if (n < 0) n = 0;

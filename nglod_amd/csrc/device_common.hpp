@@ -130,7 +130,10 @@ __device__ __forceinline__ void corner_fma(float (&f)[kC], const float* __restri
 
 // Sum over levels 0..lod of the trilinear samples (OctreeSDF.py:102-110),
 // FP32, features held in registers.
-__device__ __forceinline__ void gather_fp32(const ModelArgs& m, int lod, float px, float py,
+// LV: any table with res[] and grid[] per level (ModelArgs, or a copy of
+// them in LDS).
+template <typename LV>
+__device__ __forceinline__ void gather_fp32(const LV& m, int lod, float px, float py,
                                             float pz, float (&f)[kC]) {
 #pragma unroll
     for (int c = 0; c < kC; ++c) f[c] = 0.0f;

@@ -342,6 +342,447 @@ __global__ __launch_bounds__(kMarchBlock) void normals_pass_kernel(TraceArgs a) 
     add_evals(a, 0, nevals, lane);
 }
 
+// ---------------------------------------------------------------------------
+// Pooled trace: the rays of several requests (each its own model and LOD) in
+// one launch per pass.  The rays live in one pool of per-ray arrays; request
+// r owns pool slots [off_r, off_r + count_r) and virtual ids
+// [vbeg_r, vbeg_r + count_r) of the launch's id space, which the ray feed
+// hands out (XCD ranges and chunk claims as for one frame).  A lane keeps its
+// ray's request; each wave trip evaluates the busy lanes request by request
+// (usually one group; two where a wave's rays straddle a request boundary),
+// with a decoder slot per wave in LDS that is restaged when the group's
+// request differs from the one it holds.  Records passed between passes are
+// indexed by pool slot.  Per-ray arithmetic is that of the single-request
+// kernels, so a request's output does not depend on the other requests.
+
+// Index of the request that owns virtual id v (requests sorted by vbeg).
+__device__ __forceinline__ int pool_request_of(const PoolArgs& p, int64_t v) {
+    int lo = 0, hi = p.nreq - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (p.req[mid].vbeg <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int64_t pool_slot_of(const PoolArgs& p, int r, int64_t v) {
+    return p.req[r].off + (v - p.req[r].vbeg);
+}
+
+// Orders a wave's LDS writes before its own later LDS reads (one wave, no
+// block barrier: the waves of a block run different requests).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One model's level table in the device model table, read through the
+// constant address space: the grid pointers and resolutions come in with
+// scalar loads, as a single-request kernel reads them from its arguments.
+struct ScalarInts {
+    const int* p;
+    __device__ __forceinline__ int operator[](int i) const { return ((const NGLOD_CONST int*)p)[i]; }
+};
+template <typename T>
+struct ScalarPtrs {
+    const T* const* p;
+    __device__ __forceinline__ const T* operator[](int i) const { return ((const T* const NGLOD_CONST*)p)[i]; }
+};
+struct PoolLevelsH {  // FP16 grids (gather_fp16)
+    ScalarInts res;
+    ScalarPtrs<uint16_t> grid_h;
+};
+struct PoolLevels32 {  // FP32 grids (gather_fp32)
+    ScalarInts res;
+    ScalarPtrs<float> grid;
+};
+__device__ __forceinline__ PoolLevelsH pool_levels_h(const PoolArgs& p, int g) {
+    const ModelArgs* m = p.models + p.req[g].model;
+    return PoolLevelsH{ScalarInts{m->res}, ScalarPtrs<uint16_t>{m->grid_h}};
+}
+__device__ __forceinline__ PoolLevels32 pool_levels32(const PoolArgs& p, int g) {
+    const ModelArgs* m = p.models + p.req[g].model;
+    return PoolLevels32{ScalarInts{m->res}, ScalarPtrs<float>{m->grid}};
+}
+
+// A wave's LDS slot in the pooled mixed march: the decoder, the level
+// table of the request it belongs to (for the cooperative tail gather, which
+// reads MixedLevels as vector values), and b1.
+struct PoolMixedSlot {
+    MixedLds dec;
+    MixedLevels lv;
+    float b1;
+};
+
+// A wave's LDS slot in the pooled FP32 passes: decoder, level table, b1 and
+// the one-query-at-a-time tail's hidden vector.
+struct PoolFp32Slot {
+    Fp32Lds dec;
+    SoloLevels lv;
+    float b1;
+    __align__(16) float sv[kH];
+};
+
+// Fills a wave's mixed slot for request e (the wave's 64 lanes).  Out of
+// line: restaging is rare, and its code inlined into the march loop would
+// weigh on the loop's registers and instruction cache.
+__device__ __noinline__ void stage_pool_mixed(const ModelArgs* mp, int dec, PoolMixedSlot* s, uint32_t lane) {
+    const ModelArgs& m = *mp;
+    const int din = m.pos_inv ? kC : kC + 3;
+    const uint4* frag = reinterpret_cast<const uint4*>(m.w0_bf16[dec]);
+    for (int j = (int)lane; j < 4 * 2 * 64; j += 64) s->dec.wfrag[j] = frag[j];
+    for (int j = (int)lane; j < kH; j += 64) {
+        s->dec.b0[j] = m.b0[dec][j];
+        s->dec.w1[j] = m.w1[dec][j];
+        const bool xyz = !m.pos_inv;
+        s->dec.wxyz[j][0] = xyz ? m.w0[dec][j * din + 0] : 0.0f;
+        s->dec.wxyz[j][1] = xyz ? m.w0[dec][j * din + 1] : 0.0f;
+        s->dec.wxyz[j][2] = xyz ? m.w0[dec][j * din + 2] : 0.0f;
+        s->dec.wxyz[j][3] = 0.0f;
+    }
+    if (lane < (uint32_t)NGLOD_MAX_LODS) {
+        s->lv.grid_h[lane] = reinterpret_cast<const __half*>(m.grid_h[lane]);
+        s->lv.res[lane] = m.res[lane];
+    }
+    if (lane == 0) s->b1 = m.b1[dec][0];
+}
+
+// Fills a wave's FP32 slot for request e (the wave's 64 lanes); out of line.
+__device__ __noinline__ void stage_pool_fp32(const ModelArgs* mp, int dec, PoolFp32Slot* s, uint32_t lane) {
+    const ModelArgs& m = *mp;
+    const float* W0 = m.w0[dec];
+    const int din = m.pos_inv ? kC : kC + 3;
+    const int off = m.pos_inv ? 0 : 3;
+    for (int j = (int)lane; j < 4 * 4 * 64; j += 64) {
+        const int l = j & 63;
+        const int kq = (j >> 6) & 3;
+        const int h = j >> 8;
+        const int row = 32 * h + (l & 31);
+        const int col = off + 8 * kq + (l >> 5);
+        float4 v;
+        v.x = W0[row * din + col + 0];
+        v.y = W0[row * din + col + 2];
+        v.z = W0[row * din + col + 4];
+        v.w = W0[row * din + col + 6];
+        s->dec.afrag[j] = v;
+    }
+    for (int j = (int)lane; j < kH; j += 64) {
+        s->dec.b0[j] = m.b0[dec][j];
+        s->dec.w1[j] = m.w1[dec][j];
+        const bool xyz = !m.pos_inv;
+        s->dec.wxyz[j][0] = xyz ? W0[j * din + 0] : 0.0f;
+        s->dec.wxyz[j][1] = xyz ? W0[j * din + 1] : 0.0f;
+        s->dec.wxyz[j][2] = xyz ? W0[j * din + 2] : 0.0f;
+        s->dec.wxyz[j][3] = 0.0f;
+    }
+    if (lane < (uint32_t)NGLOD_MAX_LODS) {
+        s->lv.grid[lane] = m.grid[lane];
+        s->lv.res[lane] = m.res[lane];
+    }
+    if (lane == 0) s->b1 = m.b1[dec][0];
+}
+
+// Makes the wave's slot hold request g (restaging it if it holds another).
+template <typename SLOT, typename STAGE>
+__device__ __forceinline__ void hold_request(const PoolArgs& p, int g, int& held, SLOT* slot, uint32_t lane,
+                                             STAGE stage) {
+    if (g == held) return;
+    wave_lds_sync();
+    stage(p.models + p.req[g].model, p.req[g].dec, slot, lane);
+    wave_lds_sync();
+    held = g;
+}
+
+constexpr int kPoolFp32Block = 256;
+
+// Three waves per SIMD, as the single-request march (its registers allow
+// three; the group loop would otherwise push this kernel to two).
+template <bool POSINV>
+__global__ __launch_bounds__(kMixedBlock, 3) void pool_pass1_kernel(TraceArgs a, PoolArgs p) {
+    __shared__ PoolMixedSlot slots[kMixedBlock / 64];
+    PoolMixedSlot* sl = &slots[threadIdx.x >> 6];
+    const uint32_t lane = lane_id();
+    int held = -1;  // request whose decoder this wave's slot holds (wave-uniform)
+    RayFeed feed;
+    feed.init(p.total);
+    RayState s;
+    s.ray = -1;
+    s.phase = PH_IDLE;
+    int rq = 0;
+    uint32_t nevals = 0;
+    MarchHistory hist;
+    hist.clear();
+    const int coop = a.solo_lanes < kCoopLanes ? a.solo_lanes : kCoopLanes;
+    bool tail = false;
+    for (;;) {
+        for (;;) {
+            const bool need = s.phase == PH_IDLE;
+            if (__ballot(need && !feed.empty()) == 0) break;
+            const int64_t v = feed.next<kPass1Chunk>(a.queue, nullptr, lane, need);
+            if (v >= 0) {
+                rq = pool_request_of(p, v);
+                s.start(a, pool_slot_of(p, rq, v));
+            }
+        }
+        const uint64_t act = __ballot(s.phase != PH_IDLE);
+        if (act == 0) break;
+        if (feed.empty() && __popcll(act) <= coop) {
+            tail = true;
+            break;
+        }
+        const bool busy = s.phase != PH_IDLE;
+        float d = 0.0f;
+        uint64_t todo = act;
+        while (todo) {
+            const int g = __builtin_amdgcn_readlane(rq, __ffsll((unsigned long long)todo) - 1);
+            const bool in = busy && rq == g;
+            todo &= ~__ballot(in);
+            hold_request(p, g, held, sl, lane, stage_pool_mixed);
+            const float dg = sdf_mixed_wave<POSINV>(pool_levels_h(p, g), p.req[g].lod, &sl->dec, sl->b1, lane, in,
+                                                    s.qx, s.qy, s.qz);
+            if (in) d = dg;
+        }
+        if (busy) {
+            ++nevals;
+            hist.push(s);
+            s.consume(a, d, true);
+            if (s.phase == PH_REPLAY) {
+                hist.rewind(s, a);
+                s.hand_off_to(a.handoff, a.hflag);
+            }
+        }
+    }
+    // the wave's last few rays: the gather spread over the idle lanes, group by group
+    while (tail) {
+        const uint64_t act = __ballot(s.phase != PH_IDLE);
+        if (act == 0) break;
+        const bool busy = s.phase != PH_IDLE;
+        float d = 0.0f;
+        uint64_t todo = act;
+        while (todo) {
+            const int g = __builtin_amdgcn_readlane(rq, __ffsll((unsigned long long)todo) - 1);
+            const bool in = busy && rq == g;
+            const uint64_t gm = __ballot(in);
+            todo &= ~gm;
+            hold_request(p, g, held, sl, lane, stage_pool_mixed);
+            __half2 acc[kC / 2];
+            gather_fp16_coop(&sl->lv, p.req[g].lod, lane, gm, s.qx, s.qy, s.qz, acc);
+            const float dg = decode_mixed_wave<POSINV>(acc, &sl->dec, sl->b1, lane, in, s.qx, s.qy, s.qz);
+            if (in) d = dg;
+        }
+        if (busy) {
+            ++nevals;
+            hist.push(s);
+            s.consume(a, d, true);
+            if (s.phase == PH_REPLAY) {
+                hist.rewind(s, a);
+                s.hand_off_to(a.handoff, a.hflag);
+            }
+        }
+    }
+    add_evals(a, 1, nevals, lane);
+}
+
+// The next flagged record for each lane with `need`: the wave claims 64
+// virtual ids at a time, each lane loads the flag of its id's pool slot, and
+// the flagged ids are handed out in order from the ballot mask (as
+// RayFeed::next_flagged, through the slot map).  Returns the pool slot (-1:
+// none left) and the request in r.
+template <bool SPLIT>
+__device__ __forceinline__ int64_t pool_next_flagged(RayFeed& f, const PoolArgs& p, uint32_t* counters,
+                                                     const uint8_t* flag, uint32_t lane, bool need, int& r) {
+    int64_t slot = -1;
+    for (;;) {
+        const bool want = need && slot < 0;
+        const uint64_t mask = __ballot(want);
+        if (mask == 0) break;
+        if (f.fmask == 0) {
+            if (f.drained) break;
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            if (!f.claim<SPLIT>(counters, lane, leader, 64)) continue;
+            const int64_t c = f.cb + (int64_t)lane;
+            bool fl = false;
+            if (c < f.ce) {
+                const int rr = pool_request_of(p, c);
+                fl = flag[pool_slot_of(p, rr, c)] != 0;
+            }
+            f.fmask = __ballot(fl);
+            if (f.fmask == 0) continue;
+        }
+        const int avail = __popcll(f.fmask);
+        const int cnt = __popcll(mask);
+        if (want) {
+            const int k = (int)mask_rank(mask);
+            if (k < avail) {
+                const int64_t v = f.cb + nth_set_bit(f.fmask, k);
+                r = pool_request_of(p, v);
+                slot = pool_slot_of(p, r, v);
+            }
+        }
+        if (cnt >= avail)
+            f.fmask = 0;
+        else
+            f.fmask &= ~((1ull << nth_set_bit(f.fmask, cnt)) - 1ull);
+    }
+    return slot;
+}
+
+// Pass 2 over the pool: FP32 continuation of the handed-off rays, claimed in
+// virtual-id order (flags indexed by pool slot), with the one-query-at-a-time
+// tail of the single-request pass.
+template <bool POSINV>
+__global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a, PoolArgs p) {
+    __shared__ PoolFp32Slot slots[kPoolFp32Block / 64];
+    PoolFp32Slot* sl = &slots[threadIdx.x >> 6];
+    const uint32_t lane = lane_id();
+    int held = -1;
+    RayFeed feed;
+    feed.init(p.total);
+    RayState s;
+    s.ray = -1;
+    s.phase = PH_IDLE;
+    int rq = 0;
+    uint32_t nevals = 0;
+    bool tail = false;
+    for (;;) {
+        int r = 0;
+        const int64_t slot = pool_next_flagged<true>(feed, p, a.handoff_queue, a.hflag, lane, s.phase == PH_IDLE, r);
+        if (slot >= 0) {
+            rq = r;
+            s.resume(a, a.handoff[slot]);
+        }
+        const uint64_t act = __ballot(s.phase != PH_IDLE);
+        if (act == 0) break;
+        if (feed.drained && feed.fmask == 0 && __popcll(act) <= a.solo_lanes) {
+            tail = true;
+            break;
+        }
+        const bool busy = s.phase != PH_IDLE;
+        float d = 0.0f;
+        uint64_t todo = act;
+        while (todo) {
+            const int g = __builtin_amdgcn_readlane(rq, __ffsll((unsigned long long)todo) - 1);
+            const bool in = busy && rq == g;
+            todo &= ~__ballot(in);
+            hold_request(p, g, held, sl, lane, stage_pool_fp32);
+            const float dg = sdf_fp32_wave_lv<POSINV>(pool_levels32(p, g), p.req[g].lod, &sl->dec, sl->b1, lane, in,
+                                                      s.qx, s.qy, s.qz);
+            if (in) d = dg;
+        }
+        if (busy) {
+            ++nevals;
+            s.consume(a, d);
+        }
+    }
+    // the wave's last few rays, one query at a time with the whole wave
+    while (tail) {
+        const uint64_t act = __ballot(s.phase != PH_IDLE);
+        if (act == 0) break;
+        float d = 0.0f;
+        uint64_t mm = act;
+        while (mm) {
+            const int q = __ffsll((unsigned long long)mm) - 1;
+            mm &= mm - 1ull;
+            const int g = __builtin_amdgcn_readlane(rq, q);
+            hold_request(p, g, held, sl, lane, stage_pool_fp32);
+            const float px = __shfl(s.qx, q), py = __shfl(s.qy, q), pz = __shfl(s.qz, q);
+            const float dq = sdf_fp32_solo<POSINV>(&sl->lv, p.req[g].lod, &sl->dec, sl->b1, lane, sl->sv, px, py, pz,
+                                                   CoarsePlan{0, 0}, nullptr);
+            if ((int)lane == q) d = dq;
+        }
+        if (s.phase != PH_IDLE) {
+            ++nevals;
+            s.consume(a, d);
+        }
+    }
+    add_evals(a, 0, nevals, lane);
+}
+
+// Normals over the pool: FP32 central differences of every hit.
+template <bool POSINV>
+__global__ __launch_bounds__(kPoolFp32Block) void pool_normals_kernel(TraceArgs a, PoolArgs p) {
+    __shared__ PoolFp32Slot slots[kPoolFp32Block / 64];
+    PoolFp32Slot* sl = &slots[threadIdx.x >> 6];
+    const uint32_t lane = lane_id();
+    int held = -1;
+    uint32_t nevals = 0;
+    bool drained = false;
+    for (;;) {
+        // one virtual id per idle lane from one counter, in order
+        int64_t slot = -1;
+        int rq = 0;
+        for (;;) {
+            const bool want = slot < 0 && !drained;
+            const uint64_t mask = __ballot(want);
+            if (mask == 0) break;
+            const int cnt = __popcll(mask);
+            const int leader = __ffsll((unsigned long long)mask) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(a.normal_queue, (uint32_t)cnt);
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
+            if (want) {
+                const int64_t v = (int64_t)base + mask_rank(mask);
+                if (v < p.total) {
+                    const int r = pool_request_of(p, v);
+                    const int64_t sv = pool_slot_of(p, r, v);
+                    if (a.hit[sv]) {
+                        slot = sv;
+                        rq = r;
+                    }
+                }
+            }
+            if ((int64_t)base + cnt >= p.total) drained = true;
+        }
+        const bool valid = slot >= 0;
+        const uint64_t act = __ballot(valid);
+        if (act == 0) break;
+        float px = 0.0f, py = 0.0f, pz = 0.0f;
+        if (valid) {
+            px = a.x[3 * slot];
+            py = a.x[3 * slot + 1];
+            pz = a.x[3 * slot + 2];
+        }
+        float g3[3] = {0.0f, 0.0f, 0.0f};
+        uint64_t todo = act;
+        while (todo) {
+            const int g = __builtin_amdgcn_readlane(rq, __ffsll((unsigned long long)todo) - 1);
+            const bool in = valid && rq == g;
+            todo &= ~__ballot(in);
+            hold_request(p, g, held, sl, lane, stage_pool_fp32);
+            const int lod = p.req[g].lod;
+            const PoolLevels32 lv = pool_levels32(p, g);
+#pragma unroll 1
+            for (int ax = 0; ax < 3; ++ax) {
+                float pp[3] = {px + 0.0f, py + 0.0f, pz + 0.0f}, pm[3] = {px - 0.0f, py - 0.0f, pz - 0.0f};
+                const float c = ax == 0 ? px : (ax == 1 ? py : pz);
+                pp[ax] = c + kFdH;
+                pm[ax] = c - kFdH;
+                const float fp = sdf_fp32_wave_lv<POSINV>(lv, lod, &sl->dec, sl->b1, lane, in, pp[0], pp[1], pp[2]);
+                const float fm = sdf_fp32_wave_lv<POSINV>(lv, lod, &sl->dec, sl->b1, lane, in, pm[0], pm[1], pm[2]);
+                if (in) g3[ax] = (fp - fm) / kFdTwoH;
+            }
+        }
+        if (valid) {
+            nevals += 6;
+            normalize_grad(g3[0], g3[1], g3[2], a.normal[3 * slot], a.normal[3 * slot + 1], a.normal[3 * slot + 2]);
+        }
+    }
+    add_evals(a, 0, nevals, lane);
+}
+
+// Clears the hand-off flags of the launch's slots (one thread per virtual id)
+// and the claim counters.
+__global__ void pool_clear_kernel(PoolArgs p, uint8_t* hflag, uint32_t* counters, int ncounters) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < ncounters) counters[v] = 0u;
+    if (v < p.total) {
+        const int r = pool_request_of(p, v);
+        hflag[pool_slot_of(p, r, v)] = 0;
+    }
+}
+
 template <typename K>
 static int grid_for(K kernel, int64_t n, int block) {
     int per_cu = 0;
@@ -426,6 +867,39 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
 
 int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& timer) {
     return a.m.pos_inv ? trace_mixed_rounds<true>(a, n, s, timer) : trace_mixed_rounds<false>(a, n, s, timer);
+}
+
+template <bool POSINV>
+static int trace_pool_passes(const TraceArgs& a, const PoolArgs& p, hipStream_t s, PassTimer& timer) {
+    const int64_t n = p.total;
+    hipLaunchKernelGGL(pool_clear_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, a.hflag, a.queue,
+                       4 * kQueues);
+    int rc = check_launch("pool_clear_kernel");
+    if (rc) return rc;
+    timer.mark(kPassOther);
+    hipLaunchKernelGGL(pool_pass1_kernel<POSINV>, dim3(grid_for(pool_pass1_kernel<POSINV>, n, kMixedBlock)),
+                       dim3(kMixedBlock), 0, s, a, p);
+    rc = check_launch("pool_pass1_kernel");
+    if (rc) return rc;
+    timer.mark(kPassMarch);
+    hipLaunchKernelGGL(pool_pass2_kernel<POSINV>, dim3(grid_for(pool_pass2_kernel<POSINV>, n, kPoolFp32Block)),
+                       dim3(kPoolFp32Block), 0, s, a, p);
+    rc = check_launch("pool_pass2_kernel");
+    if (rc) return rc;
+    timer.mark(kPassFp32);
+    if (a.normals) {
+        hipLaunchKernelGGL(pool_normals_kernel<POSINV>,
+                           dim3(grid_for(pool_normals_kernel<POSINV>, n, kPoolFp32Block)), dim3(kPoolFp32Block), 0, s,
+                           a, p);
+        rc = check_launch("pool_normals_kernel");
+        if (rc) return rc;
+        timer.mark(kPassNormals);
+    }
+    return NGLOD_OK;
+}
+
+int launch_trace_pool(const TraceArgs& a, const PoolArgs& p, bool pos_inv, hipStream_t s, PassTimer& timer) {
+    return pos_inv ? trace_pool_passes<true>(a, p, s, timer) : trace_pool_passes<false>(a, p, s, timer);
 }
 
 }  // namespace nglod

@@ -66,7 +66,10 @@ __device__ __forceinline__ void corner_hfma(__half2 (&acc)[kC / 2], const __half
     }
 }
 
-__device__ __forceinline__ void gather_fp16(const ModelArgs& m, int lod, float px, float py, float pz,
+// LV: any table with res[] and grid_h[] per level (ModelArgs, or
+// MixedLevels in LDS).
+template <typename LV>
+__device__ __forceinline__ void gather_fp16(const LV& m, int lod, float px, float py, float pz,
                                             __half2 (&acc)[kC / 2]) {
 #pragma unroll
     for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
@@ -195,8 +198,8 @@ __device__ __forceinline__ float decode_mixed_wave(const __half2 (&acc)[kC / 2],
     return (lo ? part0 : part1) + b1;
 }
 
-template <bool POSINV>
-__device__ __forceinline__ float sdf_mixed_wave(const ModelArgs& m, int lod, const MixedLds* s, float b1,
+template <bool POSINV, typename LV>
+__device__ __forceinline__ float sdf_mixed_wave(const LV& m, int lod, const MixedLds* s, float b1,
                                                 uint32_t lane, bool active, float px, float py, float pz) {
     __half2 acc[kC / 2];
     if (active) {

@@ -2,6 +2,8 @@
 // persistent trace kernels.
 #pragma once
 
+#include <stdlib.h>
+
 #include "coarse_lds.hpp"
 #include "device_common.hpp"
 #include "fp32_wave.hpp"
@@ -94,6 +96,14 @@ struct TraceArgs {
 
 // Default of TraceArgs::solo_lanes; NGLOD_TRACE_SOLO_LANES (0..64) overrides it.
 constexpr int kSoloLanesDefault = 4;
+
+// TraceArgs::solo_lanes: NGLOD_TRACE_SOLO_LANES (0 = off .. 64) or the default.
+inline int solo_lanes_setting() {
+    const char* e = getenv("NGLOD_TRACE_SOLO_LANES");
+    if (!e || !*e) return kSoloLanesDefault;
+    const int v = atoi(e);
+    return v < 0 ? 0 : (v > 64 ? 64 : v);
+}
 
 struct RayState {
     int64_t ray;

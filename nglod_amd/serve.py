@@ -4,18 +4,26 @@ The reference has no serving layer; its nearest analogues are the batch loop
 of Renderer.render (sdf-net/lib/renderer.py:118-123) and the per-frame driver
 of the SOL viewer (sol-renderer/sdfRenderer.cpp:217-242).
 
-Design:
-- Models stay resident with their mixed-precision buffers packed once.
-- One device byte arena (the ray pool) holds every live request's rays,
-  outputs and trace workspace; a request is admitted when a slot (a HIP
-  stream) is free and the pool has room for its bytes, in arrival order.
-- A request's work is one camera-ray launch and one persistent trace call
-  (nglod_trace_dense) on its slot's stream, so requests that arrive while
-  others run start as soon as a slot and bytes free up, and their kernels
-  share the device (a small frame's waves fill the tail of a large one).
-- Completion is seen by polling each request's event; its pool range and
-  slot are released at once.  Per-ray arithmetic does not depend on which
-  other rays are in flight, so a result is the same alone and in the mix.
+Two modes share the admission rules (arrival order, capped by ray-pool
+memory) and report the same statistics:
+
+- mode="pooled" (default): the pool is one set of per-ray arrays; a request
+  takes a range of ray slots.  Every admitted request that is waiting when a
+  launch slot frees up joins the next batch, and a batch is one
+  nglod_trace_pool call: one persistent mixed-precision march over the rays
+  of all its requests (each with its own model and LOD), one FP32
+  continuation pass and one normals pass.  Up to `slots` batches are in
+  flight, each on its own stream, so requests that arrive while a batch runs
+  start in a new batch at once when a batch slot is free, and the passes of
+  two batches overlap on the device (one batch's drain tail under the
+  other's march).
+- mode="streams": each request is its own nglod_trace_dense call on one of
+  `slots` HIP streams, so the persistent kernels of up to `slots` requests
+  share the device.
+
+Completion is seen by polling each batch's (request's) event; its pool range
+is released at once.  Per-ray arithmetic does not depend on which other rays
+are in flight, so a result is the same alone and in the mix, in either mode.
 """
 import ctypes
 import math
@@ -69,9 +77,9 @@ class RequestResult:
 class RayPool:
     """First-fit allocator over one device byte buffer (256-byte aligned)."""
 
-    def __init__(self, nbytes, device):
+    def __init__(self, nbytes, device, allocate=True):
         self.nbytes = _up(int(nbytes))
-        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device) if allocate else None
         self.free_list = [(0, self.nbytes)]  # sorted (offset, size)
         self.used = {}
 
@@ -115,6 +123,7 @@ class ServeStats:
     wall_s: float = 0.0
     peak_pool_bytes: int = 0
     peak_inflight: int = 0
+    launches: int = 0  # trace launches (one per request, or one per pooled batch)
 
     @property
     def rays(self):
@@ -134,6 +143,7 @@ class ServeStats:
             "p99_ms": self.latency_ms(99),
             "peak_pool_bytes": self.peak_pool_bytes,
             "peak_inflight": self.peak_inflight,
+            "launches": self.launches,
         }
 
 
@@ -145,13 +155,16 @@ class TraceServer:
 
     def __init__(self, models, pool_bytes=1 << 30, slots=4, form="mixed", num_steps=128, min_dis=3e-4,
                  far_clip=10.0, step_size=1.0, refine_steps=4, return_dist=0.0, near_dist=1e-2,
-                 keep_outputs=False):
+                 keep_outputs=False, mode="pooled"):
+        if mode not in ("pooled", "streams"):
+            raise ValueError(f"mode must be 'pooled' or 'streams', got {mode!r}")
+        if mode == "pooled" and (form != "mixed" or return_dist > 0.0):
+            raise ValueError("the pooled mode traces in the mixed form with return_dist = 0")
+        self.mode = mode
         self.models = list(models)
         self.device = self.models[0].features[0].fm.device
         self.form = _lib.MIXED if form == "mixed" else _lib.FP32
         self.descs = [m.descriptor(self.form) for m in self.models]  # parameters fixed while serving
-        self.pool = RayPool(pool_bytes, self.device)
-        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(slots)]
         self.keep_outputs = keep_outputs
         p = _lib.NglodTraceParams()
         p.num_steps = int(num_steps)
@@ -164,11 +177,52 @@ class TraceServer:
         p.near_dist = float(near_dist)
         self.params = p
         self.lib = _lib.load()
+        self.slots = int(slots)
+        if mode == "streams":
+            self.pool = RayPool(pool_bytes, self.device)
+            self.streams = [torch.cuda.Stream(device=self.device) for _ in range(slots)]
+        else:
+            self._init_pooled(pool_bytes)
+
+    # bytes per ray slot of the pooled mode: o, d, x, normal (12 each), t (4),
+    # hit (1), and the trace workspace's hand-off record and flag (48 + 1)
+    _POOLED_BYTES_PER_RAY = 4 * 12 + 4 + 1 + 49
+
+    def _init_pooled(self, pool_bytes):
+        lib = self.lib
+        cap = int(pool_bytes) // self._POOLED_BYTES_PER_RAY // _ALIGN * _ALIGN
+        if cap < _ALIGN:
+            raise ValueError("pool_bytes too small for the pooled mode")
+        dev = self.device
+        self.pool_rays = cap
+        self.pool = RayPool(cap, dev, allocate=False)  # allocator over ray slots
+        self.ro = torch.empty(cap, 3, device=dev)
+        self.rd = torch.empty(cap, 3, device=dev)
+        self.px = torch.empty(cap, 3, device=dev)
+        self.pt = torch.empty(cap, 1, device=dev)
+        self.phit = torch.empty(cap, dtype=torch.bool, device=dev)
+        self.pn = torch.empty(cap, 3, device=dev)
+        self.pws = torch.empty(int(lib.nglod_trace_pool_workspace_size(cap)), dtype=torch.uint8, device=dev)
+        # one stream and one set of claim counters per batch in flight
+        self.bstreams = [torch.cuda.Stream(device=dev) for _ in range(self.slots)]
+        self.counters = torch.empty(self.slots, 256, dtype=torch.uint8, device=dev)
+        n = len(self.descs)
+        self.desc_array = (_lib.NglodModel * n)(*self.descs)
+        self.table = torch.empty(int(lib.nglod_model_table_bytes(n)), dtype=torch.uint8, device=dev)
+        _lib.check(lib.nglod_model_table_fill(self.desc_array, n, _lib.ptr(self.table), self.table.numel(),
+                                              _lib.stream_of(self.table)), "nglod_model_table_fill")
 
     def request_bytes(self, req):
+        """Ray-pool bytes a request holds while it is in flight."""
         n = req.rays
+        if self.mode == "pooled":
+            return _up(n) * self._POOLED_BYTES_PER_RAY
         ws = int(self.lib.nglod_trace_dense_workspace_size(n, self.form))
         return sum(_up(b) for b in (12 * n, 12 * n, 12 * n, 4 * n, n, 12 * n, ws))
+
+    @property
+    def pool_nbytes(self):
+        return self.pool.nbytes * (self._POOLED_BYTES_PER_RAY if self.mode == "pooled" else 1)
 
     def _launch(self, req, slot, off):
         n = req.rays
@@ -209,10 +263,12 @@ class TraceServer:
         """Serves `requests` (arrival times in seconds from now) to completion."""
         reqs = sorted(requests, key=lambda r: (r.arrival, r.rid))
         for r in reqs:
-            if self.request_bytes(r) > self.pool.nbytes:
+            if self.request_bytes(r) > self.pool_nbytes:
                 raise ValueError(f"request {r.rid} ({r.width}x{r.height}) needs more than the whole ray pool")
             if not 0 <= r.model < len(self.models):
                 raise ValueError(f"request {r.rid}: no model {r.model}")
+        if self.mode == "pooled":
+            return self._run_pooled(reqs)
         stats = ServeStats()
         free_slots = list(range(len(self.streams)))
         waiting = []
@@ -246,11 +302,99 @@ class TraceServer:
                 slot = free_slots.pop(0)
                 start = time.perf_counter() - t0
                 ev, outs = self._launch(req, slot, off)
+                stats.launches += 1
                 inflight.append((req, slot, off, ev, start, outs))
             stats.peak_pool_bytes = max(stats.peak_pool_bytes, self.pool.bytes_in_use)
             stats.peak_inflight = max(stats.peak_inflight, len(inflight))
             if not inflight and not waiting and nxt < len(reqs):
                 # idle until the next arrival
+                delay = reqs[nxt].arrival - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(min(delay, 1e-3))
+        stats.wall_s = time.perf_counter() - t0 - (reqs[0].arrival if reqs else 0.0)
+        stats.results.sort(key=lambda r: r.rid)
+        return stats
+
+
+    # ---- pooled mode -------------------------------------------------------
+    def _launch_batch(self, batch, k):
+        """Camera rays of each request into its pool slots, then one
+        nglod_trace_pool call for the whole batch, on batch stream k with
+        claim counters k.  batch: [(req, slot0)]."""
+        lib = self.lib
+        stream = self.bstreams[k]
+        sp = ctypes.c_void_p(stream.cuda_stream)
+        reqs = (_lib.NglodPoolRequest * len(batch))()
+        for j, (req, off) in enumerate(batch):
+            eye = (ctypes.c_float * 3)(*req.eye)
+            at = (ctypes.c_float * 3)(*req.at)
+            _lib.check(lib.nglod_camera_rays(eye, at, float(req.fov), req.width, req.height, 0,
+                                             ctypes.c_void_p(self.ro.data_ptr() + 12 * off),
+                                             ctypes.c_void_p(self.rd.data_ptr() + 12 * off), sp), "nglod_camera_rays")
+            reqs[j].model = req.model
+            reqs[j].lod = req.lod
+            reqs[j].offset = off
+            reqs[j].count = req.rays
+        _lib.check(lib.nglod_trace_pool(self.desc_array, _lib.ptr(self.table), len(self.descs), reqs, len(batch),
+                                        self.params, _lib.ptr(self.ro), _lib.ptr(self.rd), _lib.ptr(self.px),
+                                        _lib.ptr(self.pt), _lib.ptr(self.phit), _lib.ptr(self.pn), self.pool_rays,
+                                        _lib.ptr(self.pws), self.pws.numel(), _lib.ptr(self.counters[k]), None, sp),
+                   "nglod_trace_pool")
+        outs = []
+        for req, off in batch:
+            o = None
+            if self.keep_outputs:
+                sl = slice(off, off + req.rays)
+                with torch.cuda.stream(stream):
+                    o = {"x": self.px[sl].clone(), "depth": self.pt[sl].clone(), "hit": self.phit[sl].clone(),
+                         "normal": self.pn[sl].clone(), "ray_o": self.ro[sl].clone(), "ray_d": self.rd[sl].clone()}
+            outs.append(o)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        return ev, outs
+
+    def _run_pooled(self, reqs):
+        stats = ServeStats()
+        waiting = []
+        inflight = []  # (batch [(req, off)], event, start, outputs, stream index)
+        free_k = list(range(self.slots))
+        nxt = 0
+        torch.cuda.synchronize(self.device)
+        t0 = time.perf_counter()
+        while nxt < len(reqs) or waiting or inflight:
+            now = time.perf_counter() - t0
+            while nxt < len(reqs) and reqs[nxt].arrival <= now:
+                waiting.append(reqs[nxt])
+                nxt += 1
+            still = []
+            for item in inflight:
+                batch, ev, start, outs, k = item
+                if ev.query():
+                    fin = time.perf_counter() - t0
+                    for (req, off), o in zip(batch, outs):
+                        self.pool.free(off)
+                        stats.results.append(RequestResult(req.rid, req.arrival, start, fin, req.rays, o))
+                    free_k.append(k)
+                else:
+                    still.append(item)
+            inflight = still
+            # the next batch: every waiting request, in arrival order, that fits the pool
+            if waiting and free_k:
+                batch = []
+                while waiting and len(batch) < 64:
+                    off = self.pool.alloc(waiting[0].rays)
+                    if off is None:
+                        break
+                    batch.append((waiting.pop(0), off))
+                if batch:
+                    k = free_k.pop(0)
+                    start = time.perf_counter() - t0
+                    ev, outs = self._launch_batch(batch, k)
+                    stats.launches += 1
+                    inflight.append((batch, ev, start, outs, k))
+            stats.peak_pool_bytes = max(stats.peak_pool_bytes, self.pool.bytes_in_use * self._POOLED_BYTES_PER_RAY)
+            stats.peak_inflight = max(stats.peak_inflight, sum(len(b[0]) for b in inflight))
+            if not inflight and not waiting and nxt < len(reqs):
                 delay = reqs[nxt].arrival - (time.perf_counter() - t0)
                 if delay > 0:
                     time.sleep(min(delay, 1e-3))

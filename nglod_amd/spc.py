@@ -57,6 +57,7 @@ class SPC:
         self.pyramid = torch.tensor([list(pyr[0:L.value + 2]), list(pyr[L.value + 2:2 * (L.value + 2)])],
                                     dtype=torch.int32)
         self._pyr_c = (ctypes.c_int32 * (2 * (self.level + 2)))(*self.pyramid.reshape(-1).tolist())
+        self._ws_async = {}
 
     @classmethod
     def from_mesh(cls, tris, level, device="cuda"):
@@ -127,6 +128,42 @@ class SPC:
             break
         _lib.check(rc, "nglod_spc_raytrace")
         return nug[:count.value], offs
+
+    def raytrace_async(self, ray_o, ray_d, target, capacity=None, out=None):
+        """The single-pass traversal without a host read-back (targets up to
+        9): returns (nuggets [capacity, 2] int32, offsets [N+1] int32,
+        status [2] int32 on the device: the nugget count, and flags, bit 0 =
+        more nuggets than `capacity` (the list holds the first `capacity`;
+        offsets are complete), bit 1 = traversal error).  Rows past the count
+        are not written.  capacity defaults to 8 per ray; `out` reuses a
+        (nuggets, offsets, status) triple of the right sizes."""
+        _lib.require_gpu(ray_o, ray_d)
+        o = ray_o.detach().float().contiguous()
+        d = ray_d.detach().float().contiguous()
+        n = o.shape[0]
+        dev = o.device
+        lib = _lib.load()
+        if out is None:
+            cap = 8 * n if capacity is None else int(capacity)
+            nug = torch.empty(max(cap, 1), 2, dtype=torch.int32, device=dev)
+            offs = torch.empty(n + 1, dtype=torch.int32, device=dev)
+            status = torch.empty(2, dtype=torch.int32, device=dev)
+        else:
+            nug, offs, status = out
+            cap = nug.shape[0] if capacity is None else int(capacity)
+            if offs.numel() < n + 1 or status.numel() < 2 or cap > nug.shape[0]:
+                raise ValueError("raytrace_async: `out` buffers too small")
+        need = lib.nglod_spc_raytrace_async_workspace_size(n)
+        ws = self._ws_async.get(dev)
+        if ws is None or ws.numel() < need:
+            ws = torch.empty(max(need, 256), dtype=torch.uint8, device=dev)
+            self._ws_async[dev] = ws
+        _lib.check(lib.nglod_spc_raytrace_async(_lib.ptr(self.octree), _lib.ptr(self.prefix), _lib.ptr(self.points),
+                                                ctypes.cast(self._pyr_c, ctypes.c_void_p), self.level, _lib.ptr(o),
+                                                _lib.ptr(d), n, target, _lib.ptr(nug), cap, _lib.ptr(offs),
+                                                _lib.ptr(status), _lib.ptr(ws), ws.numel(), _lib.stream_of(o)),
+                   "nglod_spc_raytrace_async")
+        return nug, offs, status
 
     @staticmethod
     def mark_unique_rays(nuggets):

@@ -70,3 +70,69 @@ def test_raytrace_random_rays_and_edge_cases(dev, shape):
         np.testing.assert_array_equal(nug.cpu().numpy(), want)
     empty, offs = spc.raytrace(torch.zeros(0, 3, device=dev), torch.zeros(0, 3, device=dev), 3)
     assert empty.shape == (0, 2)
+
+
+def _oracle_offsets(want, n):
+    cnt = np.bincount(want[:, 0], minlength=n) if want.size else np.zeros(n, np.int64)
+    return np.concatenate([[0], np.cumsum(cnt)])
+
+
+def test_raytrace_dense_octree_uses_the_fallback(dev):
+    """Every cell of level 6 occupied: a 64-ray tile hits far more cells than
+    the LDS lists hold, so its rays take the per-ray depth-first fallback;
+    the list is still the oracle's, bit for bit (tiles on the fast path
+    mixed in: rays that miss the box)."""
+    from tests.helpers import full_octree
+    octree = full_octree(6)
+    spc = SPC(torch.from_numpy(octree), dev)
+    o, d = workloads.camera_rays(80, 70, eye=(0.4, 2.2, 2.5), fov=60.0, device="cpu")
+    o, d = o.numpy(), d.numpy()
+    for target in (4, 6):
+        nug, offs = spc.raytrace(torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev), target)
+        want = oracle.spc_raytrace(octree, spc.points.cpu().numpy(), spc.pyramid.numpy(), 6, o, d, target)
+        if target == 6:  # dense: tiles whose lists outgrow the 2048-entry LDS buffers
+            assert (np.bincount(want[:, 0] // 64) > 2048).sum() > 10
+        np.testing.assert_array_equal(nug.cpu().numpy(), want)
+        np.testing.assert_array_equal(offs.cpu().numpy(), _oracle_offsets(want, o.shape[0]))
+
+
+def test_raytrace_async_capacity_and_count_only(dev, shape):
+    """nglod_spc_raytrace_async on the 1280x720 frame (level-7 octree,
+    target 6): count-only (capacity 0), a cut list (capacity half the count:
+    flag set, the first rows and every offset right) and a full list."""
+    spc = SPC.from_mesh(torch.from_numpy(shape), 7, dev)
+    o, d = workloads.camera_rays(1280, 720, device="cpu")
+    want = oracle.spc_raytrace(spc.octree.cpu().numpy(), spc.points.cpu().numpy(), spc.pyramid.numpy(), 7,
+                               o.numpy(), d.numpy(), 6)
+    woffs = _oracle_offsets(want, o.shape[0])
+    og, dg = o.to(dev), d.to(dev)
+    total = want.shape[0]
+    for cap in (0, total // 2, total, total + 1000):
+        nug, offs, status = spc.raytrace_async(og, dg, 6, capacity=cap)
+        torch.cuda.synchronize()
+        st = status.cpu().numpy()
+        assert st[0] == total, (cap, st)
+        assert st[1] == (1 if cap < total else 0), (cap, st)
+        np.testing.assert_array_equal(offs.cpu().numpy(), woffs)
+        k = min(cap, total)
+        np.testing.assert_array_equal(nug[:k].cpu().numpy(), want[:k])
+
+
+def test_raytrace_deep_target_two_pass(dev):
+    """Targets deeper than the single pass (here 10) take the two-pass
+    depth-first traversal: a small triangle's level-10 cells (the occupancy
+    builder tests a triangle's bounding-box cells in one thread, so the
+    triangle is kept small), rays aimed at it."""
+    tri = np.array([[[-0.07, -0.06, 0.01], [0.06, -0.05, -0.02], [0.005, 0.07, 0.03]]], np.float32)
+    spc = SPC.from_mesh(torch.from_numpy(tri), 10, dev)
+    rng = np.random.default_rng(9)
+    n = 3000
+    o = np.tile(np.array([[0.01, 0.02, 3.0]], np.float32), (n, 1))
+    tgt = rng.uniform(-0.05, 0.05, size=(n, 3)) * np.array([1.0, 1.0, 0.2])
+    d = tgt - o
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    nug, offs = spc.raytrace(torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev), 10)
+    want = oracle.spc_raytrace(spc.octree.cpu().numpy(), spc.points.cpu().numpy(), spc.pyramid.numpy(), 10, o, d, 10)
+    assert want.shape[0] > 1000
+    np.testing.assert_array_equal(nug.cpu().numpy(), want)
+    np.testing.assert_array_equal(offs.cpu().numpy(), _oracle_offsets(want, n))

@@ -29,13 +29,13 @@ def _loss_grads_gpu(net, pts, gts):
     return loss.item(), {k: v.grad.detach().clone() for k, v in net.named_parameters() if v.grad is not None}
 
 
-def _loss_grads_cpu64(net, pts, gts):
-    ref = TorchOctreeSDF.from_module(net, dtype=torch.float64)
+def _loss_grads_cpu64(net, pts, gts, dtype=torch.float64):
+    ref = TorchOctreeSDF.from_module(net, dtype=dtype)
     params = ref.fms + ref.w0 + ref.b0 + ref.w1 + ref.b1
     for p in params:
         p.requires_grad_(True)
-    x = pts.detach().cpu().double()
-    gt = gts.detach().cpu().double()
+    x = pts.detach().cpu().to(dtype)
+    gt = gts.detach().cpu().to(dtype)
     preds = ref.sdf(x, return_lst=True)
     loss = sum(((p - gt) ** 2).sum() for p in preds) / x.shape[0]
     loss.backward()
@@ -166,8 +166,12 @@ def test_training_step_full_batch_vs_cpu_autograd_fp64(dev):
     near the surface at sigma 0.01 and 0.003) with FP64 GPU mesh2sdf targets,
     L2 loss summed over 5 LODs (sdf-net/lib/trainer.py:317-339).  Targets on
     a strided sample against the FP64 oracle; the five LODs' gradients
-    against FP64 CPU autograd within 1e-4 relative; two runs bit-identical;
-    one Adam step moves every parameter."""
+    against FP64 CPU autograd, within 1e-4 of each tensor's largest entry or
+    within twice the deviation of the reference's own FP32 CPU autograd
+    (the same graph in FP32), whichever is larger: at 2^19 samples a grid
+    entry sums thousands of FP32 contributions of both signs, so FP32
+    rounding alone can exceed 1e-4 of the largest entry; two runs
+    bit-identical; one Adam step moves every parameter."""
     V, F = mesh.displaced_icosphere(0)
     assert F.shape[0] == 5120
     md = MeshData(V, F, dev)
@@ -184,14 +188,21 @@ def test_training_step_full_batch_vs_cpu_autograd_fp64(dev):
     for k in gg:
         assert torch.equal(gg[k], gg2[k]), k
     lc, gc = _loss_grads_cpu64(net, pts, gts)
+    _, g32 = _loss_grads_cpu64(net, pts, gts, dtype=torch.float32)
     assert abs(lg - lc) < 1e-5 * max(1.0, abs(lc))
     assert set(gg) == set(gc)
+    report = {}
     for k in gc:
         a = gg[k].double().cpu().reshape(gc[k].shape)
         b = gc[k]
         scale = b.abs().max().item()
         assert scale > 0.0, k
-        assert (a - b).abs().max().item() <= 1e-4 * scale, (k, (a - b).abs().max().item(), scale)
+        err = (a - b).abs().max().item()
+        err_ref32 = (g32[k].double().reshape(b.shape) - b).abs().max().item()
+        report[k] = (err / scale, err_ref32 / scale)
+        assert err <= max(1e-4 * scale, 2.0 * err_ref32), (k, err, err_ref32, scale)
+    print("relative max error (GPU, reference FP32 CPU):",
+          {k: (f"{v[0]:.1e}", f"{v[1]:.1e}") for k, v in report.items()})
     tr = trainer.Trainer(net)
     before = {k: v.detach().clone() for k, v in net.named_parameters()}
     tr.step_geometry(pts, gts)

@@ -77,8 +77,9 @@ def _reqs():
 
 
 @pytest.mark.gpu
-def test_same_result_alone_and_in_the_mix(models):
-    server = TraceServer(models, pool_bytes=64 << 20, slots=4, keep_outputs=True)
+@pytest.mark.parametrize("mode", ["pooled", "streams"])
+def test_same_result_alone_and_in_the_mix(models, mode):
+    server = TraceServer(models, pool_bytes=64 << 20, slots=4, keep_outputs=True, mode=mode)
     reqs = _reqs()
     mix = server.run(reqs)
     assert len(mix.results) == len(reqs) and mix.peak_inflight > 1
@@ -99,21 +100,23 @@ def test_same_result_alone_and_in_the_mix(models):
 
 
 @pytest.mark.gpu
-def test_admission_is_capped_by_pool_bytes(models):
-    probe = TraceServer(models, pool_bytes=1 << 20, slots=4)
+@pytest.mark.parametrize("mode", ["pooled", "streams"])
+def test_admission_is_capped_by_pool_bytes(models, mode):
+    probe = TraceServer(models, pool_bytes=1 << 20, slots=4, mode=mode)
     one = probe.request_bytes(RenderRequest(0, 256, 256, 4, (0.0, 0.0, 3.0)))
-    server = TraceServer(models, pool_bytes=one + one // 2, slots=4)
+    server = TraceServer(models, pool_bytes=one + one // 2, slots=4, mode=mode)
     reqs = [RenderRequest(i % 3, 256, 256, 4, (0.0, 0.0, 3.0), rid=i) for i in range(6)]
     st = server.run(reqs)
     assert len(st.results) == 6
-    assert st.peak_inflight == 1 and st.peak_pool_bytes <= server.pool.nbytes
+    assert st.peak_inflight == 1 and st.peak_pool_bytes <= server.pool_nbytes
     with pytest.raises(ValueError):
         server.run([RenderRequest(0, 1280, 720, 4, (0.0, 0.0, 3.0))])
 
 
 @pytest.mark.gpu
-def test_poisson_stream_latencies(models):
-    server = TraceServer(models, pool_bytes=256 << 20, slots=4)
+@pytest.mark.parametrize("mode", ["pooled", "streams"])
+def test_poisson_stream_latencies(models, mode):
+    server = TraceServer(models, pool_bytes=256 << 20, slots=4, mode=mode)
     reqs = request_stream(40, 200.0, len(models), seed=1, resolutions=((64, 64), (256, 256), (640, 480)))
     st = server.run(reqs)
     s = st.summary()
@@ -147,9 +150,9 @@ def test_serving_workload_full_size(fitted_models):
     for i in range(10):
         w, h = RESOLUTIONS[i % 5]
         reqs.append(RenderRequest(model=i % 8, width=w, height=h, lod=(3 * i) % 5, eye=orbit_eye(rng), rid=i))
-    server = TraceServer(fitted_models, pool_bytes=2 << 30, slots=4, keep_outputs=True)
+    server = TraceServer(fitted_models, pool_bytes=2 << 30, slots=2, keep_outputs=True, mode="pooled")
     mix = server.run(reqs)
-    assert len(mix.results) == len(reqs) and mix.peak_inflight > 1
+    assert len(mix.results) == len(reqs) and mix.peak_inflight > 1 and mix.launches < len(reqs)
     for req, res in zip(reqs, mix.results):
         alone = server.run([RenderRequest(**{**req.__dict__, "arrival": 0.0})]).results[0]
         for k in ("x", "depth", "hit", "normal"):
@@ -167,3 +170,9 @@ def test_serving_workload_full_size(fitted_models):
     assert both.sum() > 100
     derr = np.abs(out["depth"][:, 0].cpu().numpy()[idx][both] - ref["t"][both])
     assert derr.mean() <= 1e-3, derr.mean()
+    # the pooled batch gives the same bits as one trace call per request on its own stream
+    streams = TraceServer(fitted_models, pool_bytes=2 << 30, slots=4, keep_outputs=True, mode="streams")
+    other = streams.run(reqs)
+    for req, a, b in zip(reqs, mix.results, other.results):
+        for k in ("x", "depth", "hit", "normal"):
+            assert torch.equal(a.outputs[k], b.outputs[k]), (req.rid, k)
