@@ -167,6 +167,28 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
                       void* workspace, size_t workspace_bytes, uint64_t* evals_out,
                       int32_t* ray_steps, void* stream);
 
+/* Most rays one nglod_trace_dense_noalloc call takes in NGLOD_MIXED (2^20: a
+ * 1280x720 frame; the FP32 form has no such limit). */
+#define NGLOD_NOALLOC_MAX_MIXED_RAYS (1 << 20)
+
+/* No counterpart in the reference.  For callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): nglod_trace_dense without a workspace argument; the same
+ * kernels and the same outputs, bit for bit.  Its scratch (claim counters,
+ * and for NGLOD_MIXED the hand-off records and flags) is a file-scope device
+ * array of 256 B + 49 B * NGLOD_NOALLOC_MAX_MIXED_RAYS (about 49 MiB) that each
+ * device holds from the library's load until the process ends.  The call
+ * allocates nothing.  Limits: NGLOD_MIXED takes n <= NGLOD_NOALLOC_MAX_MIXED_RAYS
+ * (larger n: NGLOD_ERR_CAPACITY; rays are independent, so a larger frame can
+ * be traced in chunks with the same results) and needs params->return_dist = 0
+ * (else NGLOD_ERR_UNSUPPORTED).  Every call on a device shares that device's
+ * array and clears what must start at zero: calls that overlap on one device
+ * (concurrent streams or threads) race. */
+int nglod_trace_dense_noalloc(const nglod_model_t* model, const float* ray_o, const float* ray_d,
+                              int64_t n, int32_t lod, const nglod_trace_params_t* params, int32_t form,
+                              float* x, float* t, uint8_t* hit, float* normal, uint64_t* evals_out,
+                              int32_t* ray_steps, void* stream);
+
 /* ---- pooled trace (serving) -------------------------------------------- */
 
 /* No counterpart in the reference (its nearest analogue is the batch loop of

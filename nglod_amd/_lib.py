@@ -122,6 +122,11 @@ SIGNATURES = {
         [ctypes.POINTER(NglodModel), _p, _p, _i64, _i32, ctypes.POINTER(NglodTraceParams), _i32,
          _p, _p, _p, _p, _p, _sz, _p, _p, _p],
     ),
+    "nglod_trace_dense_noalloc": (
+        ctypes.c_int,
+        [ctypes.POINTER(NglodModel), _p, _p, _i64, _i32, ctypes.POINTER(NglodTraceParams), _i32,
+         _p, _p, _p, _p, _p, _p, _p],
+    ),
     "nglod_normals_fd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _p, _p]),
     "nglod_pack_mixed": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p]),
     "nglod_sdf_bwd_workspace_size": (_sz, [ctypes.POINTER(NglodModel), _i64, _i32]),

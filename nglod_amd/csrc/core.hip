@@ -8,6 +8,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <mutex>
+
 #include "api_common.hpp"
 #include "fp32_wave.hpp"
 #include "trace_common.hpp"
@@ -321,12 +323,13 @@ size_t nglod_trace_dense_workspace_size(int64_t n, int32_t form) {
     return kWsCounters + (form == NGLOD_MIXED ? (size_t)n * 2 * sizeof(Handoff) + 2 * flag_bytes(n) : 0);
 }
 
-int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const float* ray_d, int64_t n, int32_t lod,
-                      const nglod_trace_params_t* params, int32_t form, float* x, float* t, uint8_t* hit,
-                      float* normal, void* workspace, size_t workspace_bytes, uint64_t* evals_out,
-                      int32_t* ray_steps, void* stream) {
-    ModelArgs m;
-    int rc = make_model_args(model, lod, form, &m);
+}  // extern "C"
+
+// Argument checks shared by the trace entry points (before any workspace).
+static int check_trace_args(const nglod_model_t* model, const float* ray_o, const float* ray_d, int64_t n,
+                            int32_t lod, const nglod_trace_params_t* params, int32_t form, float* x, float* t,
+                            uint8_t* hit, float* normal, ModelArgs* m) {
+    int rc = make_model_args(model, lod, form, m);
     if (rc) return rc;
     if (!params) return fail(NGLOD_ERR_ARG, "params is NULL");
     if (params->num_steps < 1) return fail(NGLOD_ERR_ARG, "num_steps must be >= 1");
@@ -334,12 +337,27 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
         return fail(NGLOD_ERR_ARG, "refine_steps must be 1..4");
     if (form == NGLOD_MIXED && !(params->near_dist >= 0.0)) return fail(NGLOD_ERR_ARG, "near_dist must be >= 0");
     if (n < 0 || n > (int64_t)0x7fff0000) return fail(NGLOD_ERR_ARG, "n out of range");
-    if (n == 0) return NGLOD_OK;
-    if (!ray_o || !ray_d || !x || !t || !hit || !normal) return fail(NGLOD_ERR_ARG, "NULL pointer");
-    if (!workspace || workspace_bytes < nglod_trace_dense_workspace_size(n, form))
-        return fail(NGLOD_ERR_WORKSPACE, "workspace too small");
-    if (!aligned16(workspace)) return fail(NGLOD_ERR_ARG, "workspace not 16-byte aligned");
-    hipStream_t s = (hipStream_t)stream;
+    if (n > 0 && (!ray_o || !ray_d || !x || !t || !hit || !normal)) return fail(NGLOD_ERR_ARG, "NULL pointer");
+    return NGLOD_OK;
+}
+
+// Scratch of one trace call: kWsCounters bytes of claim counters and, for
+// the mixed form, the hand-off records and flags [n] and (when near misses
+// may go back to mixed precision) the hand-back records and flags [n].
+struct TraceScratch {
+    uint32_t* counters;
+    Handoff* handoff;
+    Handoff* ret;
+    uint8_t* hflag;
+    uint8_t* rflag;
+};
+
+// The trace on checked arguments (n > 0) and laid-out scratch.
+static int trace_dense_run(const nglod_model_t* model, const ModelArgs& m, const float* ray_o, const float* ray_d,
+                           int64_t n, int32_t lod, const nglod_trace_params_t* params, int32_t form, float* x,
+                           float* t, uint8_t* hit, float* normal, const TraceScratch& w, uint64_t* evals_out,
+                           int32_t* ray_steps, hipStream_t s) {
+    int rc = NGLOD_OK;
     TraceArgs a;
     a.m = m;
     a.lod = lod;
@@ -356,21 +374,14 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     a.t = t;
     a.hit = hit;
     a.normal = normal;
-    a.queue = (uint32_t*)workspace;
+    a.queue = w.counters;
     a.handoff_queue = a.queue + kQueues;
     a.ret_queue = a.queue + 2 * kQueues;
     a.normal_queue = a.queue + 3 * kQueues;
-    a.handoff = nullptr;
-    a.ret = nullptr;
-    a.hflag = nullptr;
-    a.rflag = nullptr;
-    if (form == NGLOD_MIXED) {
-        char* w = (char*)workspace + kWsCounters;
-        a.handoff = (Handoff*)w;
-        a.ret = (Handoff*)(w + (size_t)n * sizeof(Handoff));
-        a.hflag = (uint8_t*)(w + 2 * (size_t)n * sizeof(Handoff));
-        a.rflag = a.hflag + flag_bytes(n);
-    }
+    a.handoff = w.handoff;
+    a.ret = w.ret;
+    a.hflag = w.hflag;
+    a.rflag = w.rflag;
     a.return_dist = (float)params->return_dist;
     // The FP32 form takes its normal taps in-lane: a separate normals pass
     // measured 3-4% slower for it.
@@ -404,4 +415,91 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
     return timer.finish();
 }
 
-}  // extern "C"
+extern "C" int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const float* ray_d, int64_t n,
+                                 int32_t lod, const nglod_trace_params_t* params, int32_t form, float* x, float* t,
+                                 uint8_t* hit, float* normal, void* workspace, size_t workspace_bytes,
+                                 uint64_t* evals_out, int32_t* ray_steps, void* stream) {
+    ModelArgs m;
+    int rc = check_trace_args(model, ray_o, ray_d, n, lod, params, form, x, t, hit, normal, &m);
+    if (rc) return rc;
+    if (n == 0) return NGLOD_OK;
+    if (!workspace || workspace_bytes < nglod_trace_dense_workspace_size(n, form))
+        return fail(NGLOD_ERR_WORKSPACE, "workspace too small");
+    if (!aligned16(workspace)) return fail(NGLOD_ERR_ARG, "workspace not 16-byte aligned");
+    TraceScratch w{(uint32_t*)workspace, nullptr, nullptr, nullptr, nullptr};
+    if (form == NGLOD_MIXED) {
+        char* p = (char*)workspace + kWsCounters;
+        w.handoff = (Handoff*)p;
+        w.ret = (Handoff*)(p + (size_t)n * sizeof(Handoff));
+        w.hflag = (uint8_t*)(p + 2 * (size_t)n * sizeof(Handoff));
+        w.rflag = w.hflag + flag_bytes(n);
+    }
+    return trace_dense_run(model, m, ray_o, ray_d, n, lod, params, form, x, t, hit, normal, w, evals_out, ray_steps,
+                           (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// nglod_trace_dense_noalloc: the same trace with its scratch in a file-scope
+// device array instead of a caller workspace, for callers that cannot
+// allocate (graph capture, tight serving loops).  Layout: claim counters,
+// then hand-off records [kNoAllocMaxMixedRays], then hand-off flags.  The
+// mixed form's hand-back records are not provided, so it requires
+// return_dist = 0 (one round).  Every call on a device shares that device's
+// array: calls that overlap on one device race.
+static constexpr int64_t kNoAllocMaxMixedRays = NGLOD_NOALLOC_MAX_MIXED_RAYS;
+static constexpr size_t kNoAllocHandoffBytes = (size_t)kNoAllocMaxMixedRays * sizeof(Handoff);
+static constexpr size_t kNoAllocBytes = kWsCounters + kNoAllocHandoffBytes + (size_t)kNoAllocMaxMixedRays;
+static_assert(kWsCounters % 256 == 0 && kNoAllocHandoffBytes % 256 == 0, "scratch sections stay 256-byte aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on each device, looked up once per device with
+// that device current (each device holds its own copy of the array).
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[64] = {nullptr};
+
+static int scratch_on_current_device(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64)
+        return fail(NGLOD_ERR_HIP, "nglod_trace_dense_noalloc: no current device (or device index >= 64)");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            char buf[256];
+            snprintf(buf, sizeof(buf), "nglod_trace_dense_noalloc: hipGetSymbolAddress(scratch): %s",
+                     hipGetErrorString(e));
+            return fail(NGLOD_ERR_HIP, buf);
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return NGLOD_OK;
+}
+
+extern "C" int nglod_trace_dense_noalloc(const nglod_model_t* model, const float* ray_o, const float* ray_d,
+                                         int64_t n, int32_t lod, const nglod_trace_params_t* params, int32_t form,
+                                         float* x, float* t, uint8_t* hit, float* normal, uint64_t* evals_out,
+                                         int32_t* ray_steps, void* stream) {
+    ModelArgs m;
+    int rc = check_trace_args(model, ray_o, ray_d, n, lod, params, form, x, t, hit, normal, &m);
+    if (rc) return rc;
+    if (form == NGLOD_MIXED && n > kNoAllocMaxMixedRays)
+        return fail(NGLOD_ERR_CAPACITY, "nglod_trace_dense_noalloc: the mixed form takes at most "
+                                        "NGLOD_NOALLOC_MAX_MIXED_RAYS rays per call");
+    if (form == NGLOD_MIXED && params->return_dist > 0.0)
+        return fail(NGLOD_ERR_UNSUPPORTED, "nglod_trace_dense_noalloc: the mixed form needs return_dist = 0");
+    if (n == 0) return NGLOD_OK;
+    void* base = nullptr;
+    rc = scratch_on_current_device(&base);
+    if (rc) return rc;
+    unsigned char* p = (unsigned char*)base;
+    TraceScratch w{(uint32_t*)p, nullptr, nullptr, nullptr, nullptr};
+    if (form == NGLOD_MIXED) {
+        w.handoff = (Handoff*)(p + kWsCounters);
+        w.hflag = p + kWsCounters + kNoAllocHandoffBytes;
+    }
+    return trace_dense_run(model, m, ray_o, ray_d, n, lod, params, form, x, t, hit, normal, w, evals_out, ray_steps,
+                           (hipStream_t)stream);
+}
