@@ -7,7 +7,7 @@ of the SOL viewer (sol-renderer/sdfRenderer.cpp:217-242).
 Two modes share the admission rules (arrival order, capped by ray-pool
 memory) and report the same statistics:
 
-- mode="pooled" (default): the pool is one set of per-ray arrays; a request
+- mode="pooled": the pool is one set of per-ray arrays; a request
   takes a range of ray slots.  Every admitted request that is waiting when a
   launch slot frees up joins the next batch, and a batch is one
   nglod_trace_pool call: one persistent mixed-precision march over the rays
@@ -17,9 +17,12 @@ memory) and report the same statistics:
   start in a new batch at once when a batch slot is free, and the passes of
   two batches overlap on the device (one batch's drain tail under the
   other's march).
-- mode="streams": each request is its own nglod_trace_dense call on one of
-  `slots` HIP streams, so the persistent kernels of up to `slots` requests
-  share the device.
+- mode="streams" (default): each request is its own nglod_trace_dense call on
+  one of `slots` HIP streams, so the persistent kernels of up to `slots`
+  requests share the device.  It measured faster than the pooled mode at
+  every load on the 8-model stream (saturation ≈575 vs ≈430 M rays/s): the
+  pooled march's per-trip request grouping costs more than the drain tails
+  it saves.
 
 Completion is seen by polling each batch's (request's) event; its pool range
 is released at once.  Per-ray arithmetic does not depend on which other rays
@@ -155,7 +158,7 @@ class TraceServer:
 
     def __init__(self, models, pool_bytes=1 << 30, slots=4, form="mixed", num_steps=128, min_dis=3e-4,
                  far_clip=10.0, step_size=1.0, refine_steps=4, return_dist=0.0, near_dist=1e-2,
-                 keep_outputs=False, mode="pooled"):
+                 keep_outputs=False, mode="streams"):
         if mode not in ("pooled", "streams"):
             raise ValueError(f"mode must be 'pooled' or 'streams', got {mode!r}")
         if mode == "pooled" and (form != "mixed" or return_dist > 0.0):
