@@ -38,10 +38,13 @@ def cpu_model():
     return platform.processor()
 
 
-def bytes_per_query(lod, fdim=32, elem=4):
-    """Algorithmic bytes of one SDF query at `lod`: 8 corners x C channels per
-    level for levels 0..lod, plus 12 B of position in and 4 B of distance out."""
-    return (lod + 1) * 8 * fdim * elem + 16
+def bytes_per_query(levels, fdim=32, elem=4):
+    """Algorithmic bytes of one SDF query that gathers `levels` grids: 8
+    corners x C channels per grid, plus 12 B of position in and 4 B of
+    distance out.  The FP32 form gathers levels 0..lod (lod + 1 grids); the
+    mixed form gathers one, the cumulative grid of lod (FP16 in its march,
+    FP32 in its continuation and normals)."""
+    return levels * 8 * fdim * elem + 16
 
 
 def load_traffic(name):
@@ -163,7 +166,8 @@ def roofline(kname, kp, r, args, traffic_of, counters_of, ceiling):
         "kernel": kname,
         "kernel_ms": kp["ms"],
         "algorithmic_bytes": kp["algorithmic_bytes"],
-        "bytes_per_query": {"fp32": bytes_per_query(args.lod, elem=4), "mixed": bytes_per_query(args.lod, elem=2)},
+        "bytes_per_query": {"fp32": bytes_per_query(args.lod + 1, elem=4), "mixed": bytes_per_query(1, elem=2),
+                            "mixed_fp32_passes": bytes_per_query(1, elem=4)},
         "peak_source": f"{VL1D_BYTES_PER_CLK} B/clk/CU x {CUS} CUs x {SCLK_GHZ} GHz",
         "measured_vl1d_ceiling": ceiling,
         "hbm": hbm,
@@ -296,7 +300,8 @@ def query_bench(net, dev, lod, reps=10):
 def shade_bench(net, o, d, dev, lod, width, height, num_steps, reps=5):
     """Renderer shading of the bench frame (renderer.py:131-209, :279-307):
     ground-plane shadow rays, the shadow trace (mixed form), the blurred
-    shadow map, 40-tap FP32 ambient occlusion of every hit, matcap shading.
+    shadow map, 40-tap FP32 ambient occlusion of every hit (mixed form: taps
+    on the FP32 cumulative grid), matcap shading.
     The plane sits at the frame's lowest hit point; the matcap texture is a
     seeded 512 x 512 RGB array; shadow-ray jitter off."""
     import numpy as np
@@ -333,8 +338,8 @@ def shade_bench(net, o, d, dev, lod, width, height, num_steps, reps=5):
     shadow = srb.hit & light_hit
     ms_map = timed(lambda: shadow_map(shadow, shadow, width, height))
     smap = shadow_map(shadow, shadow, width, height)
-    ms_ao = timed(lambda: ambient_occlusion(net, rb.x, rb.normal, rb.hit, plane, lod=lod))
-    ao = ambient_occlusion(net, rb.x, rb.normal, rb.hit, plane, lod=lod)
+    ms_ao = timed(lambda: ambient_occlusion(net, rb.x, rb.normal, rb.hit, plane, lod=lod, form="mixed"))
+    ao = ambient_occlusion(net, rb.x, rb.normal, rb.hit, plane, lod=lod, form="mixed")
     ms_matcap = timed(lambda: matcap_shade(d, rb.normal, rb.hit, tex, shadow=smap, ao=ao))
     torch.cuda.synchronize()
     n_hit, n_plane = int(rb.hit.sum().item()), int(plane.sum().item())
@@ -343,7 +348,7 @@ def shade_bench(net, o, d, dev, lod, width, height, num_steps, reps=5):
         "ground_y": ground, "hits": n_hit, "plane_hits": n_plane, "shadowed": int(shadow.sum().item()),
         "shadow_setup_ms": ms_setup, "shadow_trace_ms": ms_strace, "shadow_map_ms": ms_map,
         "ao_ms": ms_ao, "ao_taps": taps, "ao_taps_per_s": taps / (ms_ao * 1e-3),
-        "ao_algorithmic_GBs": taps * bytes_per_query(lod, elem=4) / (ms_ao * 1e-3) / 1e9,
+        "ao_algorithmic_GBs": taps * bytes_per_query(1, elem=4) / (ms_ao * 1e-3) / 1e9,
         "matcap_ms": ms_matcap,
         "total_ms": ms_setup + ms_strace + ms_map + ms_ao + ms_matcap,
     }
@@ -497,7 +502,9 @@ def main():
         kernel_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
         ev = tracer.last_evals.cpu().tolist()  # [FP32-form evals, mixed-form evals] of the last frame
         hits = int(rb.hit.sum().item())
-        b32, b16 = bytes_per_query(args.lod, elem=4), bytes_per_query(args.lod, elem=2)
+        # FP32 form: lod + 1 grids per query; mixed form: one cumulative grid
+        b32 = bytes_per_query(args.lod + 1 if form == "fp32" else 1, elem=4)
+        b16 = bytes_per_query(1, elem=2)
         alg = ev[0] * b32 + ev[1] * b16 + n * (24 + 29)
         # Per-pass kernel times from HIP events the library records between its
         # launches (a second run of the same frames; these calls wait for

@@ -34,7 +34,7 @@ extern "C" {
 
 /* arithmetic forms */
 #define NGLOD_FP32 0  /* FP32 grids, FP32 decoder */
-#define NGLOD_MIXED 1 /* FP16 grids, BF16 MFMA decoder with FP32 accumulation */
+#define NGLOD_MIXED 1 /* FP16 cumulative grids, BF16 MFMA decoder with FP32 accumulation */
 
 /* Dense multi-LOD model (OctreeSDF, sdf-net/lib/models/OctreeSDF.py:59-88).
  * Device pointers.  grid[i] is corner-major: element
@@ -42,7 +42,12 @@ extern "C" {
  * reference FeatureVolume (OctreeSDF.py:43) — i.e. torch channels_last_3d.
  * Decoder i: w0[i] [hidden][din], b0[i] [hidden], w1[i] [hidden], b1[i] [1],
  * din = feature_dim + (pos_invariant ? 0 : 3), xyz first (OctreeSDF.py:114-115).
- * grid_h[i] / w0_bf16[i] are only read by NGLOD_MIXED (see nglod_pack_mixed). */
+ * grid_cum[i] / grid_h[i] / w0_bf16[i] are only read by NGLOD_MIXED and are
+ * written by nglod_pack_mixed.  grid_cum[i] (same layout and size as grid[i])
+ * holds, at each corner of grid i, the sum over levels l <= i of grid l's
+ * trilinear sample at that corner's position, so that one trilinear sample of
+ * grid_cum[i] is the level sum OctreeSDF.sdf accumulates up to lod i
+ * (OctreeSDF.py:102-110); grid_h[i] is its FP16 rounding. */
 typedef struct nglod_model {
     int32_t num_lods;      /* feature volumes in use, 1..NGLOD_MAX_LODS */
     int32_t feature_dim;   /* C; kernels are built for 32 */
@@ -55,8 +60,9 @@ typedef struct nglod_model {
     const float* b0[NGLOD_MAX_LODS];
     const float* w1[NGLOD_MAX_LODS];
     const float* b1[NGLOD_MAX_LODS];
-    const uint16_t* grid_h[NGLOD_MAX_LODS];  /* FP16 copy of grid[i] */
+    const uint16_t* grid_h[NGLOD_MAX_LODS];  /* FP16 rounding of grid_cum[i] */
     const uint16_t* w0_bf16[NGLOD_MAX_LODS]; /* BF16 W0 in MFMA fragment order */
+    const float* grid_cum[NGLOD_MAX_LODS];   /* cumulative grid of level i, FP32 */
 } nglod_model_t;
 
 /* Sphere-tracer parameters (BaseTracer.__init__, sdf-net/lib/tracer/BaseTracer.py:29-45;
@@ -194,9 +200,9 @@ int nglod_trace_dense_noalloc(const nglod_model_t* model, const float* ray_o, co
 /* No counterpart in the reference (its nearest analogue is the batch loop of
  * Renderer.render, sdf-net/lib/renderer.py:118-123).  A device table of
  * models for nglod_trace_pool: nglod_model_table_fill validates the models
- * (packed for NGLOD_MIXED, all LODs) and copies their descriptors to `table`
- * (device, nglod_model_table_bytes(n)).  Synchronous; refill after a model's
- * buffers move. */
+ * (packed for NGLOD_MIXED, all LODs) and copies one descriptor per model and
+ * LOD to `table` (device, nglod_model_table_bytes(n) = n * NGLOD_MAX_LODS
+ * descriptors).  Synchronous; refill after a model's buffers move. */
 size_t nglod_model_table_bytes(int32_t n);
 int nglod_model_table_fill(const nglod_model_t* models, int32_t n, void* table, size_t table_bytes, void* stream);
 
@@ -243,11 +249,19 @@ int nglod_trace_pool(const nglod_model_t* models, const void* model_table, int32
 int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
                      int32_t form, float* normal, void* stream);
 
-/* Converts an FP32 model's grids to FP16 and the feature columns of W0 to the
- * BF16 MFMA fragment layout used by NGLOD_MIXED.  grid_h[i] must hold
- * (res[i]+1)^3*feature_dim uint16, w0_bf16[i] hidden*feature_dim uint16 (the
- * xyz columns of a position-dependent decoder stay FP32).  Replaces the half-precision export of SOL_NGLOD
- * (sdf-net/lib/models/SOL_NGLOD.py:33-46), kept on device. */
+/* Builds what NGLOD_MIXED reads from an FP32 model: the cumulative grids
+ * grid_cum[i] (float, (res[i]+1)^3*feature_dim) and their FP16 rounding
+ * grid_h[i] (uint16, same count), and the feature columns of W0 in the BF16
+ * MFMA fragment layout, w0_bf16[i] (hidden*feature_dim uint16; the xyz
+ * columns of a position-dependent decoder stay FP32).  A cumulative corner
+ * value is summed in FP64 (levels in order, each level's 8 corners in x, y, z
+ * order) and rounded once to FP32; grid_h is the FP16 rounding of that FP32
+ * value.  Every level's resolution must divide the resolutions of the levels
+ * after it (the reference's 2^(i+base_lod) do), so that each cell of grid i
+ * lies inside one cell of every coarser grid: then one trilinear sample of
+ * grid_cum[i] equals the sum of the per-level samples in exact arithmetic;
+ * otherwise NGLOD_ERR_UNSUPPORTED.  Replaces the half-precision export of
+ * SOL_NGLOD (sdf-net/lib/models/SOL_NGLOD.py:33-46), kept on device. */
 int nglod_pack_mixed(const nglod_model_t* model, void* stream);
 
 /* ---- sparse octree (SPC) --------------------------------------------------

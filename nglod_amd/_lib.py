@@ -54,6 +54,7 @@ class NglodModel(ctypes.Structure):
         ("b1", _p * MAX_LODS),
         ("grid_h", _p * MAX_LODS),
         ("w0_bf16", _p * MAX_LODS),
+        ("grid_cum", _p * MAX_LODS),
     ]
 
 

@@ -32,6 +32,10 @@ inline int fail(int code, const char* msg) {
 inline bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 // Validates the model for `lod` / `form` and fills the kernel argument copy.
+// NGLOD_FP32: every level 0..lod, gathered level by level.  NGLOD_MIXED: one
+// level, the cumulative grid of `lod` (grid_cum[lod] FP32, grid_h[lod] FP16)
+// presented as level 0 with grid lod's resolution; the kernels then gather
+// with lod gather_lod(form, lod) = 0 and decode with decoder_of(model, lod).
 inline int make_model_args(const nglod_model_t* m, int lod, int form, ModelArgs* a) {
     if (!m) return fail(NGLOD_ERR_ARG, "model is NULL");
     if (m->num_lods < 1 || m->num_lods > NGLOD_MAX_LODS) return fail(NGLOD_ERR_ARG, "num_lods out of range");
@@ -40,22 +44,28 @@ inline int make_model_args(const nglod_model_t* m, int lod, int form, ModelArgs*
         return fail(NGLOD_ERR_UNSUPPORTED, "kernels are built for feature_dim 32, hidden_dim 128");
     if (m->num_decoders != 1 && m->num_decoders != m->num_lods)
         return fail(NGLOD_ERR_ARG, "num_decoders must be 1 or num_lods");
+    if (form != NGLOD_FP32 && form != NGLOD_MIXED) return fail(NGLOD_ERR_ARG, "unknown form");
     memset(a, 0, sizeof(*a));
-    a->num_lods = m->num_lods;
     a->pos_inv = m->pos_invariant ? 1 : 0;
     a->num_dec = m->num_decoders;
-    for (int i = 0; i < m->num_lods; ++i) {
+    for (int i = 0; i < m->num_lods; ++i)
         if (m->res[i] < 1 || m->res[i] > 1024) return fail(NGLOD_ERR_ARG, "grid resolution out of range");
-        a->res[i] = m->res[i];
-        a->grid[i] = m->grid[i];
-        a->grid_h[i] = m->grid_h[i];
-        if (i <= lod) {
-            // the mixed form re-evaluates near the surface in FP32: it needs both grids
-            if (!m->grid[i] || !aligned16(m->grid[i]))
+    if (form == NGLOD_FP32) {
+        a->num_lods = m->num_lods;
+        for (int i = 0; i < m->num_lods; ++i) {
+            a->res[i] = m->res[i];
+            a->grid[i] = m->grid[i];
+            if (i <= lod && (!m->grid[i] || !aligned16(m->grid[i])))
                 return fail(NGLOD_ERR_ARG, "grid pointer NULL or not 16-byte aligned");
-            if (form == NGLOD_MIXED && (!m->grid_h[i] || !aligned16(m->grid_h[i])))
-                return fail(NGLOD_ERR_ARG, "fp16 grid pointer NULL or not 16-byte aligned");
         }
+    } else {
+        if (!m->grid_cum[lod] || !aligned16(m->grid_cum[lod]) || !m->grid_h[lod] || !aligned16(m->grid_h[lod]))
+            return fail(NGLOD_ERR_ARG, "cumulative grid (grid_cum / grid_h) NULL or not 16-byte aligned: "
+                                       "nglod_pack_mixed builds them");
+        a->num_lods = 1;
+        a->res[0] = m->res[lod];
+        a->grid[0] = m->grid_cum[lod];
+        a->grid_h[0] = m->grid_h[lod];
     }
     for (int i = 0; i < m->num_decoders; ++i) {
         a->w0[i] = m->w0[i];
@@ -68,11 +78,12 @@ inline int make_model_args(const nglod_model_t* m, int lod, int form, ModelArgs*
     if (form == NGLOD_MIXED) {
         const int dec = m->num_decoders == 1 ? 0 : lod;
         if (!m->w0_bf16[dec] || !aligned16(m->w0_bf16[dec])) return fail(NGLOD_ERR_ARG, "bf16 W0 not packed");
-    } else if (form != NGLOD_FP32) {
-        return fail(NGLOD_ERR_ARG, "unknown form");
     }
     return NGLOD_OK;
 }
+
+// The lod the kernels gather to for arguments built by make_model_args.
+inline int gather_lod(int form, int lod) { return form == NGLOD_MIXED ? 0 : lod; }
 
 inline int decoder_of(const nglod_model_t* m, int lod) { return m->num_decoders == 1 ? 0 : lod; }
 

@@ -269,7 +269,7 @@ int nglod_sdf_fwd(const nglod_model_t* model, const float* x, int64_t n, int32_t
     if (n == 0) return NGLOD_OK;
     if (!x || !out) return fail(NGLOD_ERR_ARG, "NULL pointer");
     const int dec = decoder_of(model, lod);
-    if (form == NGLOD_MIXED) return launch_sdf_mixed(m, lod, dec, x, n, out, nullptr, (hipStream_t)stream);
+    if (form == NGLOD_MIXED) return launch_sdf_mixed(m, 0, dec, x, n, out, nullptr, (hipStream_t)stream);
     return launch_sdf_fp32(m, lod, dec, x, n, out, nullptr, (hipStream_t)stream);
 }
 
@@ -291,10 +291,10 @@ int nglod_sdf_fwd_all(const nglod_model_t* model, const float* x, int64_t n, int
 int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod, int32_t form, float* normal,
                      void* stream) {
     // Normals are FP32 in both forms (finite differences of h = 1/192 need
-    // more than BF16 resolution); `form` only has to name a known form.
-    if (form != NGLOD_FP32 && form != NGLOD_MIXED) return fail(NGLOD_ERR_ARG, "unknown form");
+    // more than BF16 resolution); the mixed form takes them from the FP32
+    // cumulative grid, as its trace's normals pass does.
     ModelArgs m;
-    int rc = make_model_args(model, lod, NGLOD_FP32, &m);
+    int rc = make_model_args(model, lod, form, &m);
     if (rc) return rc;
     if (n < 0) return fail(NGLOD_ERR_ARG, "n < 0");
     if (n == 0) return NGLOD_OK;
@@ -302,9 +302,11 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
     const int dec = decoder_of(model, lod);
     const int64_t blocks = (n + 255) / 256;
     if (m.pos_inv)
-        hipLaunchKernelGGL(normals_fp32_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, dec, x, n, normal);
+        hipLaunchKernelGGL(normals_fp32_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m,
+                           gather_lod(form, lod), dec, x, n, normal);
     else
-        hipLaunchKernelGGL(normals_fp32_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m, lod, dec, x, n, normal);
+        hipLaunchKernelGGL(normals_fp32_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, m,
+                           gather_lod(form, lod), dec, x, n, normal);
     return check_launch("normals_fp32_kernel");
 }
 
@@ -360,7 +362,7 @@ static int trace_dense_run(const nglod_model_t* model, const ModelArgs& m, const
     int rc = NGLOD_OK;
     TraceArgs a;
     a.m = m;
-    a.lod = lod;
+    a.lod = gather_lod(form, lod);
     a.dec = decoder_of(model, lod);
     a.num_steps = params->num_steps;
     a.normals = params->normals ? 1 : 0;
@@ -387,7 +389,7 @@ static int trace_dense_run(const nglod_model_t* model, const ModelArgs& m, const
     // measured 3-4% slower for it.
     a.defer_normals = form == NGLOD_MIXED ? 1 : 0;
     a.solo_lanes = solo_lanes_setting();
-    a.coarse = coarse_plan(m, lod);
+    a.coarse = coarse_plan(m, a.lod);
     a.replay_steps = params->refine_steps < 1 ? 1 : params->refine_steps;
     a.near_dist = (float)params->near_dist;
     a.evals = (unsigned long long*)evals_out;

@@ -1,6 +1,6 @@
 // Mixed-precision form: FP16 grids, BF16 MFMA first decoder layer with FP32
 // accumulation, and FP32 re-evaluation near the surface.
-//   nglod_pack_mixed        FP32 model -> FP16 grids + BF16 W0 fragments
+//   nglod_pack_mixed        FP32 model -> cumulative FP32 / FP16 grids + BF16 W0 fragments
 //   launch_sdf_mixed        nglod_sdf_fwd(form = NGLOD_MIXED)
 //   launch_trace_mixed      nglod_trace_dense(form = NGLOD_MIXED):
 //     pass 1  persistent mixed march.  When a ray stops on a surface test
@@ -40,9 +40,59 @@ constexpr int kPass2Block = 512;
 constexpr int kPass1Chunk = 64;
 
 // ---------------------------------------------------------------------------
-__global__ void pack_grid_kernel(const float* __restrict__ src, __half* __restrict__ dst, int64_t n) {
+// Cumulative grid of level li: at corner (cx, cy, cz) of grid li, the sum over
+// levels l <= li of grid l's trilinear sample there (OctreeSDF.py:102-110
+// sums the per-level samples of a query the same way).  With R = res[li] and
+// r = res[l], q = R / r is an integer (checked on the host), the corner sits
+// at c / q in grid l's corner units: lower corner c / q (kept <= r - 1, the
+// border convention of axis_weights, weights (0, 1) at c = R) and weights
+// (q - c % q) / q, (c % q) / q.  Summed in FP64, levels in order and each
+// level's 8 corners in x, y, z order, weight ((wx * wy) * wz); rounded once to
+// FP32, and that FP32 value rounded to FP16.  One thread per (corner, channel).
+struct CumLevels {
+    const float* grid[NGLOD_MAX_LODS];
+    int res[NGLOD_MAX_LODS];
+};
+
+__device__ __forceinline__ void cum_axis(int c, int q, int r, int& i0, double& w0, double& w1) {
+    i0 = c / q;
+    int rem = c - i0 * q;
+    if (i0 > r - 1) {
+        i0 = r - 1;
+        rem = q;
+    }
+    w0 = (double)(q - rem) / (double)q;
+    w1 = (double)rem / (double)q;
+}
+
+__global__ void cumulative_grid_kernel(CumLevels lv, int li, float* __restrict__ out32, __half* __restrict__ out16) {
+    const int R = lv.res[li];
+    const int S = R + 1;
+    const int64_t n = (int64_t)S * S * S * kC;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = __float2half_rn(src[i]);
+    if (i >= n) return;
+    const int c = (int)(i % kC);
+    const int64_t corner = i / kC;
+    const int cx = (int)(corner % S), cy = (int)((corner / S) % S), cz = (int)(corner / ((int64_t)S * S));
+    double acc = 0.0;
+    for (int l = 0; l <= li; ++l) {
+        const int r = lv.res[l], q = R / r, s = r + 1;
+        int ix, iy, iz;
+        double wx[2], wy[2], wz[2];
+        cum_axis(cx, q, r, ix, wx[0], wx[1]);
+        cum_axis(cy, q, r, iy, wy[0], wy[1]);
+        cum_axis(cz, q, r, iz, wz[0], wz[1]);
+        const float* g = lv.grid[l] + ((size_t)((iz * s + iy) * s + ix)) * kC + c;
+        const size_t sx = kC, sy = (size_t)s * kC, sz = (size_t)s * s * kC;
+        for (int k = 0; k < 8; ++k) {
+            const int bx = k & 1, by = (k >> 1) & 1, bz = k >> 2;
+            const double w = (wx[bx] * wy[by]) * wz[bz];
+            acc = acc + w * (double)g[bx * sx + by * sy + bz * sz];
+        }
+    }
+    const float f = (float)acc;
+    out32[i] = f;
+    out16[i] = __float2half_rn(f);
 }
 
 // Fragment (h, kb, lane, e) = W0[32h + lane%32][off + 16kb + 8*(lane/32) + e].
@@ -914,15 +964,26 @@ extern "C" int nglod_pack_mixed(const nglod_model_t* model, void* stream) {
     if (model->num_decoders != 1 && model->num_decoders != model->num_lods)
         return fail(NGLOD_ERR_ARG, "num_decoders must be 1 or num_lods");
     hipStream_t s = (hipStream_t)stream;
+    CumLevels lv;
+    memset(&lv, 0, sizeof(lv));
     for (int i = 0; i < model->num_lods; ++i) {
         const int r = model->res[i];
         if (r < 1 || r > 1024) return fail(NGLOD_ERR_ARG, "grid resolution out of range");
-        if (!model->grid[i] || !model->grid_h[i] || !aligned16(model->grid_h[i]))
-            return fail(NGLOD_ERR_ARG, "grid / grid_h pointer NULL or misaligned");
+        for (int l = 0; l < i; ++l)
+            if (r % model->res[l] != 0)
+                return fail(NGLOD_ERR_UNSUPPORTED, "the mixed form needs each grid resolution to divide the later ones");
+        if (!model->grid[i] || !aligned16(model->grid[i]) || !model->grid_cum[i] || !aligned16(model->grid_cum[i]) ||
+            !model->grid_h[i] || !aligned16(model->grid_h[i]))
+            return fail(NGLOD_ERR_ARG, "grid / grid_cum / grid_h pointer NULL or misaligned");
+        lv.grid[i] = model->grid[i];
+        lv.res[i] = r;
+    }
+    for (int i = 0; i < model->num_lods; ++i) {
+        const int r = model->res[i];
         const int64_t cnt = (int64_t)(r + 1) * (r + 1) * (r + 1) * kC;
-        hipLaunchKernelGGL(pack_grid_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, model->grid[i],
-                           (__half*)model->grid_h[i], cnt);
-        int rc = check_launch("pack_grid_kernel");
+        hipLaunchKernelGGL(cumulative_grid_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, lv, i,
+                           (float*)model->grid_cum[i], (__half*)model->grid_h[i]);
+        int rc = check_launch("cumulative_grid_kernel");
         if (rc) return rc;
     }
     const int din = model->pos_invariant ? kC : kC + 3;

@@ -21,22 +21,32 @@ size_t pool_flag_bytes(int64_t n) { return ((size_t)n + 255) / 256 * 256; }
 
 extern "C" {
 
-size_t nglod_model_table_bytes(int32_t n) { return n > 0 ? (size_t)n * sizeof(ModelArgs) : 0; }
+// The table holds one kernel argument block per (model, lod): entry
+// model * NGLOD_MAX_LODS + lod is make_model_args(model, lod, NGLOD_MIXED),
+// the cumulative grid of that lod as level 0 (unused lods stay zero).
+size_t nglod_model_table_bytes(int32_t n) { return n > 0 ? (size_t)n * NGLOD_MAX_LODS * sizeof(ModelArgs) : 0; }
 
 int nglod_model_table_fill(const nglod_model_t* models, int32_t n, void* table, size_t table_bytes, void* stream) {
     if (!models || !table || n < 1) return fail(NGLOD_ERR_ARG, "models / table NULL or n < 1");
     if (table_bytes < nglod_model_table_bytes(n)) return fail(NGLOD_ERR_CAPACITY, "table smaller than n models");
-    ModelArgs* host = (ModelArgs*)malloc((size_t)n * sizeof(ModelArgs));
+    const size_t entries = (size_t)n * NGLOD_MAX_LODS;
+    ModelArgs* host = (ModelArgs*)calloc(entries, sizeof(ModelArgs));
     if (!host) return fail(NGLOD_ERR_ARG, "host allocation failed");
     for (int i = 0; i < n; ++i) {
-        const int rc = make_model_args(&models[i], models[i].num_lods - 1, NGLOD_MIXED, &host[i]);
-        if (rc) {
+        if (models[i].num_lods < 1 || models[i].num_lods > NGLOD_MAX_LODS) {
             free(host);
-            return rc;
+            return fail(NGLOD_ERR_ARG, "num_lods out of range");
+        }
+        for (int l = 0; l < models[i].num_lods; ++l) {
+            const int rc = make_model_args(&models[i], l, NGLOD_MIXED, &host[(size_t)i * NGLOD_MAX_LODS + l]);
+            if (rc) {
+                free(host);
+                return rc;
+            }
         }
     }
     hipStream_t s = (hipStream_t)stream;
-    const bool ok = hipMemcpyAsync(table, host, (size_t)n * sizeof(ModelArgs), hipMemcpyHostToDevice, s) == hipSuccess &&
+    const bool ok = hipMemcpyAsync(table, host, entries * sizeof(ModelArgs), hipMemcpyHostToDevice, s) == hipSuccess &&
                     hipStreamSynchronize(s) == hipSuccess;
     free(host);
     return ok ? NGLOD_OK : fail(NGLOD_ERR_HIP, "model table copy failed");
@@ -82,8 +92,8 @@ int nglod_trace_pool(const nglod_model_t* models, const void* model_table, int32
         if (q.count == 0) continue;
         spans[nreq] = {q.offset, q.offset + q.count};
         PoolReq& r = p.req[nreq++];
-        r.model = q.model;
-        r.lod = q.lod;
+        r.model = q.model * NGLOD_MAX_LODS + q.lod;  // the (model, lod) table entry
+        r.lod = gather_lod(NGLOD_MIXED, q.lod);
         r.dec = decoder_of(&m, q.lod);
         r.vbeg = total;
         r.off = q.offset;

@@ -86,7 +86,7 @@ int nglod_sdf_fwd_ordered(const nglod_model_t* model, const float* x, int64_t n,
         hipSuccess)
         return fail(NGLOD_ERR_HIP, "sort failed");
     const int dec = decoder_of(model, lod);
-    if (form == NGLOD_MIXED) return launch_sdf_mixed(m, lod, dec, x, n, out, i_out, s);
+    if (form == NGLOD_MIXED) return launch_sdf_mixed(m, 0, dec, x, n, out, i_out, s);
     return launch_sdf_fp32(m, lod, dec, x, n, out, i_out, s);
 }
 

@@ -302,11 +302,10 @@ extern "C" size_t nglod_ao_workspace_size(void) { return 256; }
 extern "C" int nglod_ao(const nglod_model_t* model, const float* x, const float* normal, const uint8_t* hit,
                         const uint8_t* plane_hit, int64_t n, int32_t lod, int32_t form, float* ao, void* workspace,
                         size_t workspace_bytes, void* stream) {
-    // AO taps are FP32 in both forms, as the normals are; `form` only has to
-    // name a known form.
-    if (form != NGLOD_FP32 && form != NGLOD_MIXED) return fail(NGLOD_ERR_ARG, "unknown form");
+    // AO taps are FP32 in both forms, as the normals are; the mixed form
+    // takes them from the FP32 cumulative grid.
     AoArgs a;
-    int rc = make_model_args(model, lod, NGLOD_FP32, &a.m);
+    int rc = make_model_args(model, lod, form, &a.m);
     if (rc) return rc;
     if (n < 0) return fail(NGLOD_ERR_ARG, "n < 0");
     if (n > (int64_t)0xFFFFFFFFll - 1024) return fail(NGLOD_ERR_UNSUPPORTED, "n >= 2^32");
@@ -314,9 +313,9 @@ extern "C" int nglod_ao(const nglod_model_t* model, const float* x, const float*
     if (!x || !normal || !hit || !ao) return fail(NGLOD_ERR_ARG, "NULL pointer");
     if (!workspace || workspace_bytes < nglod_ao_workspace_size())
         return fail(NGLOD_ERR_WORKSPACE, "workspace smaller than nglod_ao_workspace_size()");
-    a.lod = lod;
+    a.lod = gather_lod(form, lod);
     a.dec = decoder_of(model, lod);
-    a.coarse = coarse_plan(a.m, lod);
+    a.coarse = coarse_plan(a.m, a.lod);
     a.x = x;
     a.normal = normal;
     a.hit = hit;

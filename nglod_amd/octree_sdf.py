@@ -108,7 +108,7 @@ class OctreeSDF(nn.Module):
             ) for _ in range(self.num_decoder)
         ])
         self.loss_preds = None
-        self._mixed = None  # (key, grids_h, w0_bf16) cache of the mixed form
+        self._mixed = None  # (key, cumulative grids FP32, FP16, w0_bf16) cache of the mixed form
         self._order_ws = {}  # device -> workspace of the spatially ordered batch query
 
     # -- reference API -----------------------------------------------------
@@ -175,33 +175,39 @@ class OctreeSDF(nn.Module):
                 keep.append(t)
                 getattr(m, name)[i] = t.data_ptr()
         if form == _lib.MIXED:
-            grids_h, w0s = self.mixed_buffers()
-            for i, g in enumerate(grids_h):
-                m.grid_h[i] = g.data_ptr()
+            grids_c, grids_h, w0s = self.mixed_buffers()
+            for i, (gc, gh) in enumerate(zip(grids_c, grids_h)):
+                m.grid_cum[i] = gc.data_ptr()
+                m.grid_h[i] = gh.data_ptr()
             for i, w in enumerate(w0s):
                 m.w0_bf16[i] = w.data_ptr()
-            keep += grids_h + w0s
+            keep += grids_c + grids_h + w0s
         m._keep = keep
         return m
 
     def mixed_buffers(self):
-        """FP16 grids and packed BF16 first-layer weights (the precision form),
-        rebuilt when any parameter changed since the last call."""
+        """What the precision form reads (nglod_pack_mixed): per LOD the
+        cumulative grid (at each corner, the sum of the trilinear samples of
+        that level and every coarser one) in FP32 and its FP16 rounding, and
+        the packed BF16 first-layer weights; rebuilt when any parameter changed
+        since the last call."""
         key = tuple((p.data_ptr(), p._version) for p in self.parameters())
         if self._mixed is not None and self._mixed[0] == key:
-            return self._mixed[1], self._mixed[2]
+            return self._mixed[1], self._mixed[2], self._mixed[3]
         dev = self.features[0].fm.device
+        grids_c = [torch.empty(f.fm.numel(), dtype=torch.float32, device=dev) for f in self.features]
         grids_h = [torch.empty(f.fm.numel(), dtype=torch.float16, device=dev) for f in self.features]
         w0s = [torch.empty(self.hidden_dim * self.fdim, dtype=torch.int16, device=dev) for _ in self.louts]
         m = self.descriptor(_lib.FP32)
-        for i, g in enumerate(grids_h):
-            m.grid_h[i] = g.data_ptr()
+        for i, (gc, gh) in enumerate(zip(grids_c, grids_h)):
+            m.grid_cum[i] = gc.data_ptr()
+            m.grid_h[i] = gh.data_ptr()
         for i, w in enumerate(w0s):
             m.w0_bf16[i] = w.data_ptr()
         lib = _lib.load()
         _lib.check(lib.nglod_pack_mixed(m, _lib.stream_of(grids_h[0])), "nglod_pack_mixed")
-        self._mixed = (key, grids_h, w0s)
-        return grids_h, w0s
+        self._mixed = (key, grids_c, grids_h, w0s)
+        return grids_c, grids_h, w0s
 
     # Batches of at least this many points are evaluated in Morton order
     # (nglod_sdf_fwd_ordered); the results are identical either way.

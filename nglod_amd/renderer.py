@@ -110,6 +110,8 @@ _ao_ws = {}
 def ambient_occlusion(net, x, normal, hit, plane_hit=None, lod=None, form="fp32"):
     """Ambient occlusion (renderer.py:184-209): 40 SDF taps along the normal
     of each hit (and plane hit), ao = clamp(1 - acc, 0.1, 1)^2; [N, 1].
+    The taps are FP32 in both forms; form 'mixed' takes them from the FP32
+    cumulative grid of `lod` (one grid per tap instead of lod + 1).
     The claim counter (workspace) is one per device: calls on different
     streams of one device must not overlap."""
     _lib.require_gpu(x, normal)
@@ -127,7 +129,7 @@ def ambient_occlusion(net, x, normal, hit, plane_hit=None, lod=None, form="fp32"
         ws = torch.empty(max(int(lib.nglod_ao_workspace_size()), 256), dtype=torch.uint8, device=x.device)
         _ao_ws[key] = ws
     fm = _lib.MIXED if form == "mixed" else _lib.FP32
-    _lib.check(lib.nglod_ao(net.descriptor(_lib.FP32), _lib.ptr(x), _lib.ptr(normal), _lib.ptr(h), _lib.ptr(ph), n,
+    _lib.check(lib.nglod_ao(net.descriptor(fm), _lib.ptr(x), _lib.ptr(normal), _lib.ptr(h), _lib.ptr(ph), n,
                             int(lod), fm, _lib.ptr(ao), _lib.ptr(ws), ws.numel(), _lib.stream_of(x)), "nglod_ao")
     return ao
 

@@ -404,3 +404,45 @@ def nugget_offsets(nuggets, n):
     r = np.asarray(nuggets, dtype=np.int64).reshape(-1, 2)[:, 0]
     counts = np.bincount(r, minlength=n)[:n]
     return np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+
+
+def cumulative_grids(model, prec="f32"):
+    """Per LOD i, the cumulative grid: at each corner of grid i, the sum over
+    levels l <= i of grid l's trilinear sample at that corner (the level sum of
+    sdf-net/lib/models/OctreeSDF.py:102-110 evaluated on grid i's corners).
+    Requires res[l] to divide res[i].  The corner c of grid i lies at c / q,
+    q = res[i] / res[l], in grid l's corner units: lower corner c // q (kept
+    <= res[l] - 1, weights (0, 1) at the last corner) and weights
+    (q - c % q) / q, (c % q) / q.  Summed in FP64, levels in order, each
+    level's 8 corners in x, y, z order with weight (wx * wy) * wz.
+    prec 'f64': the FP64 sums; 'f32': (FP32 rounding, FP16 rounding of that).
+    Corner-major arrays [(res+1)^3 * C]."""
+    C = model.fdim
+    out = []
+    for li, R in enumerate(model.res):
+        S = R + 1
+        acc = np.zeros((S, S, S, C), dtype=np.float64)  # [z, y, x, c]
+        c = np.arange(S)
+        for l in range(li + 1):
+            r = model.res[l]
+            if R % r:
+                raise ValueError("grid resolutions do not nest")
+            q = R // r
+            i0 = c // q
+            rem = c - i0 * q
+            over = i0 > r - 1
+            i0 = np.where(over, r - 1, i0)
+            rem = np.where(over, q, rem)
+            w = ((q - rem).astype(np.float64) / q, rem.astype(np.float64) / q)
+            g = model.grids[l].reshape(r + 1, r + 1, r + 1, C).astype(np.float64)
+            for k in range(8):
+                bx, by, bz = k & 1, (k >> 1) & 1, k >> 2
+                wk = (w[bx][None, None, :] * w[by][None, :, None]) * w[bz][:, None, None]
+                gk = g[np.ix_(i0 + bz, i0 + by, i0 + bx)]
+                acc = acc + wk[..., None] * gk
+        if prec == "f64":
+            out.append(acc.reshape(-1))
+        else:
+            f32 = acc.astype(np.float32).reshape(-1)
+            out.append((f32, f32.astype(np.float16)))
+    return out

@@ -1,5 +1,7 @@
-"""Precision form (FP16 grids, BF16 MFMA decoder, FP32 near-surface
-re-evaluation and FP32 normals) against the FP64 CPU oracle.
+"""Precision form (FP16 cumulative grids, BF16 MFMA decoder, FP32
+near-surface re-evaluation and FP32 normals on the FP32 cumulative grids)
+against the FP64 CPU oracle; the cumulative grids bit-exact against the
+oracle's.
 
 Bars: SDF max abs error <= 3e-2 and mean <= 3e-3 away from the surface,
 mean hit-depth error <= 1e-3 over rays both forms call hits, hit flags
@@ -51,6 +53,38 @@ def test_sdf_mixed_random_model_decoder_layout(dev):
     scale = np.abs(want).mean()
     assert scale > 1e-2
     assert np.abs(got - want).mean() < 0.02 * scale
+
+
+@pytest.mark.parametrize("pos_invariant", [True, False])
+def test_pack_mixed_cumulative_grids_bitexact(dev, pos_invariant):
+    """nglod_pack_mixed's cumulative grids (FP32 and FP16) equal the oracle's
+    bit for bit at the reference's resolutions 4..64."""
+    net = workloads.make_model(num_lods=5, pos_invariant=pos_invariant, seed=44, device="cpu")
+    with torch.no_grad():
+        for f in net.features:
+            f.fm.mul_(20.0)
+    want = oracle.cumulative_grids(arrays(net), "f32")
+    netg = net.to(dev)
+    grids_c, grids_h, _ = netg.mixed_buffers()
+    torch.cuda.synchronize()
+    for L, (w32, w16) in enumerate(want):
+        assert np.array_equal(grids_c[L].cpu().numpy(), w32), L
+        assert np.array_equal(grids_h[L].cpu().numpy().view(np.uint16), w16.view(np.uint16)), L
+
+
+def test_pack_mixed_refuses_grids_that_do_not_nest(dev):
+    net = workloads.make_model(num_lods=2, seed=45, device=dev)
+    m = net.descriptor(_lib.FP32)
+    bufs = [torch.empty(6 ** 3 * 32, dtype=torch.float32, device=dev) for _ in range(2)]
+    half = [torch.empty(6 ** 3 * 32, dtype=torch.float16, device=dev) for _ in range(2)]
+    w0s = [torch.empty(128 * 32, dtype=torch.int16, device=dev) for _ in range(2)]
+    for i in range(2):
+        m.grid_cum[i] = bufs[i].data_ptr()
+        m.grid_h[i] = half[i].data_ptr()
+        m.w0_bf16[i] = w0s[i].data_ptr()
+    m.res[0], m.res[1] = 3, 4  # 3 does not divide 4 (grids of at least that size back the pointers)
+    lib = _lib.load()
+    assert lib.nglod_pack_mixed(m, _lib.stream_of(bufs[0])) == -2  # NGLOD_ERR_UNSUPPORTED
 
 
 def _angle_deg(a, b):

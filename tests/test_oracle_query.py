@@ -145,3 +145,74 @@ def test_sdf_all_consistent_with_single_lod():
     allp = oracle.sdf_all(m, x, 2, "f64")
     for lod in range(3):
         np.testing.assert_array_equal(allp[:, lod], oracle.sdf(m, x, lod, "f64"))
+
+
+def _grid_sample64(g, x):
+    """F.grid_sample (align_corners=True, border) of a [C, S, S, S] FP64 grid at x [n, 3]."""
+    t = torch.from_numpy(np.ascontiguousarray(g)).double()[None]
+    xs = torch.from_numpy(np.ascontiguousarray(x)).double().reshape(1, -1, 1, 1, 3)
+    return F.grid_sample(t, xs, align_corners=True, padding_mode="border")[0, :, :, 0, 0].T.numpy()
+
+
+def _cum_points(res, seed):
+    """Random points inside and outside the box, every fine-cell boundary
+    plane of the finest grid, and the box corners."""
+    g = np.random.default_rng(seed)
+    x = [g.uniform(-1.3, 1.3, size=(3000, 3))]
+    R = res[-1]
+    planes = -1.0 + 2.0 * np.arange(R + 1) / R
+    b = g.uniform(-1.0, 1.0, size=(R + 1, 3))
+    b[:, g.integers(0, 3)] = planes
+    x.append(b)
+    x.append(np.array([[sx, sy, sz] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)], dtype=np.float64))
+    return np.concatenate(x)
+
+
+def test_cumulative_grid_sample_equals_the_level_sum():
+    """One trilinear sample of the cumulative grid of LOD L equals the sum of
+    the per-level samples of levels 0..L (F.grid_sample in FP64, the
+    reference's operator), inside the box, on cell boundaries and clamped at
+    the border; and a cumulative grid missing a level, or built with x and z
+    swapped, does not."""
+    net = small_model(num_lods=4, seed=21)  # res 4, 8, 16, 32
+    m = arrays(net)
+    cum = oracle.cumulative_grids(m, "f64")
+    x = _cum_points(m.res, 3)
+    C = m.fdim
+    grids = [g.reshape(r + 1, r + 1, r + 1, C).transpose(3, 0, 1, 2) for g, r in zip(m.grids, m.res)]
+    level_sum = 0.0
+    for L, R in enumerate(m.res):
+        level_sum = level_sum + _grid_sample64(grids[L], x)
+        cg = cum[L].reshape(R + 1, R + 1, R + 1, C).transpose(3, 0, 1, 2)
+        got = _grid_sample64(cg, x)
+        scale = np.abs(level_sum).max()
+        assert np.abs(got - level_sum).max() <= 1e-13 * scale, L
+        if L >= 1:
+            swapped = _grid_sample64(cg.transpose(0, 3, 2, 1), x)
+            assert np.abs(swapped - level_sum).max() > 1e-3 * scale
+            dropped = got - _grid_sample64(grids[0], x)  # as if level 0 were left out
+            assert np.abs(dropped - level_sum).max() > 1e-3 * scale
+
+
+def test_cumulative_grid_rounding_and_nesting():
+    """The FP32 values are the FP64 sums rounded once; the FP16 values are
+    those FP32 values rounded; on the coarse corners the cumulative grid is
+    the plain sum of the coincident grid values; grids that do not nest are
+    refused."""
+    net = small_model(num_lods=3, seed=22)  # res 4, 8, 16
+    m = arrays(net)
+    c64 = oracle.cumulative_grids(m, "f64")
+    c32 = oracle.cumulative_grids(m, "f32")
+    C = m.fdim
+    for L in range(3):
+        f32, f16 = c32[L]
+        assert np.array_equal(f32, c64[L].astype(np.float32))
+        assert np.array_equal(f16, f32.astype(np.float16))
+    # corner (4, 4, 4) of the 16-grid is corner (1, 1, 1) of the 4-grid and (2, 2, 2) of the 8-grid
+    want = sum(g.reshape(r + 1, r + 1, r + 1, C)[k, k, k].astype(np.float64)
+               for g, r, k in zip(m.grids, m.res, (1, 2, 4)))
+    assert np.array_equal(c64[2].reshape(17, 17, 17, C)[4, 4, 4], want)
+    bad = oracle.ModelArrays([np.zeros(4 ** 3 * C), np.zeros(5 ** 3 * C)], [3, 4], m.w0[:1] * 2, m.b0[:1] * 2,
+                             m.w1[:1] * 2, m.b1[:1] * 2, True)
+    with pytest.raises(ValueError):
+        oracle.cumulative_grids(bad)
