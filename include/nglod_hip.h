@@ -249,6 +249,50 @@ int nglod_trace_pool(const nglod_model_t* models, const void* model_table, int32
 int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int32_t lod,
                      int32_t form, float* normal, void* stream);
 
+/* gradient methods of diffutils.gradient (sdf-net/lib/diffutils.py:29-84) */
+#define NGLOD_GRAD_FINITEDIFF 0  /* central differences, h = 1/192 (:61-70) */
+#define NGLOD_GRAD_TETRAHEDRON 1 /* four tetrahedron taps, h = 1/192 (:38-60) */
+#define NGLOD_GRAD_AUTODIFF 2    /* d sdf / d x as torch autograd gives it (:32-37) */
+
+/* Gradient of OctreeSDF.sdf(x, lod) with respect to x: diffutils.gradient(x,
+ * net, method) (sdf-net/lib/diffutils.py:29-84).  x [n][3], grad [n][3], FP32
+ * arithmetic in both forms (NGLOD_MIXED: on the FP32 cumulative grid of lod).
+ * NGLOD_GRAD_AUTODIFF is the analytic derivative torch autograd computes:
+ * grid_sample's backward with align_corners=True, padding_mode='border' (no
+ * gradient through the border clamp: zero on and beyond the box faces along
+ * that axis), ReLU's (units with pre-activation <= 0 pass nothing) and the
+ * decoder's xyz columns for a position-dependent model.  mask: optional
+ * uint8 [n]; where 0 the output is 0 and nothing is evaluated (the tracers'
+ * normal[hit] = ..., SphereTracer.py:128-130).  normalize: 1 applies
+ * F.normalize(p=2, eps=1e-5) as SphereTracer.forward does (SphereTracer.py:129);
+ * 0 returns the raw gradient as diffutils.gradient and get_min do. */
+int nglod_gradient(const nglod_model_t* model, const float* x, int64_t n, int32_t lod, int32_t form,
+                   int32_t method, const uint8_t* mask, int32_t normalize, float* grad, void* stream);
+
+/* OctreeSDF.sdf(x, lod) and its autodiff gradient in one pass (sdf [n], may
+ * be NULL; grad [n][3]): the position gradient of the reference's autograd
+ * graph (OctreeSDF.py:94-155), used by the training backward when the query
+ * positions require grad. */
+int nglod_sdf_grad(const nglod_model_t* model, const float* x, int64_t n, int32_t lod, int32_t form, float* sdf,
+                   float* grad, void* stream);
+
+/* SphereTracer.get_min (sdf-net/lib/tracer/SphereTracer.py:134-218), FP32:
+ * the march of every ray from its box entry (from its origin, t = 0, when it
+ * has none), keeping the point min_x [n][3] that the reference's loop
+ * associates with the smallest distance seen (the comparison runs after x has
+ * moved by that distance).  x [n][3], t [n], hit [n] as the reference leaves
+ * them (hit: |t| < far_clip at the last loop iteration it ran, inside the
+ * box).  Normals are not computed here (nglod_gradient with mask = hit,
+ * normalize = 0 gives get_min's).  The batch semantics of the reference's
+ * loop (it ends when no ray marches) are kept: the result for a ray depends
+ * on when the batch's last ray stops, as in the reference.  params: num_steps
+ * (>= 1), min_dis, step_size, far_clip; the rest ignored.  workspace:
+ * nglod_trace_min_workspace_size(n) bytes, 4-byte aligned. */
+size_t nglod_trace_min_workspace_size(int64_t n);
+int nglod_trace_min(const nglod_model_t* model, const float* ray_o, const float* ray_d, int64_t n, int32_t lod,
+                    const nglod_trace_params_t* params, float* x, float* t, uint8_t* hit, float* min_x,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
 /* Builds what NGLOD_MIXED reads from an FP32 model: the cumulative grids
  * grid_cum[i] (float, (res[i]+1)^3*feature_dim) and their FP16 rounding
  * grid_h[i] (uint16, same count), and the feature columns of W0 in the BF16

@@ -22,6 +22,7 @@ ERR_WORKSPACE = -6
 
 FP32 = 0
 MIXED = 1
+GRAD_METHODS = {"finitediff": 0, "tetrahedron": 1, "autodiff": 2}
 
 _STATUS = {
     ERR_ARG: "bad argument",
@@ -130,6 +131,11 @@ SIGNATURES = {
     ),
     "nglod_normals_fd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _p, _p]),
     "nglod_pack_mixed": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p]),
+    "nglod_gradient": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _i32, _p, _i32, _p, _p]),
+    "nglod_sdf_grad": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _i32, _p, _p, _p]),
+    "nglod_trace_min_workspace_size": (_sz, [_i64]),
+    "nglod_trace_min": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _p, _i64, _i32, ctypes.POINTER(NglodTraceParams),
+                                       _p, _p, _p, _p, _p, _sz, _p]),
     "nglod_sdf_bwd_workspace_size": (_sz, [ctypes.POINTER(NglodModel), _i64, _i32]),
     "nglod_sdf_bwd": (ctypes.c_int, [ctypes.POINTER(NglodModel), _p, _i64, _i32, _p, _i32,
                                       ctypes.POINTER(NglodModelGrads), _p, _sz, _p]),

@@ -1,7 +1,11 @@
 """torch.autograd.Function around the fused HIP query (OctreeSDF.sdf) with the
-hand-written backward (nglod_sdf_bwd): parameter gradients only, computed
-deterministically (no float atomics)."""
+hand-written backward: parameter gradients by nglod_sdf_bwd, computed
+deterministically (no float atomics), and position gradients (when the query
+points require grad) by nglod_sdf_grad, the derivative torch autograd takes
+through grid_sample, the decoder and ReLU.  First order only: the backward
+itself is not differentiable (gradient penalties through it raise)."""
 import torch
+from torch.autograd.function import once_differentiable
 
 from . import _lib
 
@@ -29,15 +33,16 @@ class _SdfFunction(torch.autograd.Function):
         ctx.lod = lod
         ctx.all_levels = all_levels
         xf = x.detach().reshape(-1, 3).float().contiguous()
+        ctx.x_shape = x.shape
+        ctx.x_dtype = x.dtype
         ctx.save_for_backward(xf)
         if all_levels:
             return torch.stack(net.sdf_all_hip(xf, lod), 0)
         return net.sdf_hip(xf, lod)
 
     @staticmethod
+    @once_differentiable
     def backward(ctx, grad_out):
-        if ctx.needs_input_grad[1]:
-            raise NotImplementedError("gradients with respect to query positions are not computed")
         (x,) = ctx.saved_tensors
         net, L = ctx.net, ctx.lod
         n = x.shape[0]
@@ -49,6 +54,16 @@ class _SdfFunction(torch.autograd.Function):
             g = torch.zeros(L + 1, n, dtype=torch.float32, device=dev)
             g[L] = grad_out[:, 0].float()
             mask = 1 << L
+        grad_x = None
+        if ctx.needs_input_grad[1]:
+            # sum over the decoded levels of d loss / d d_l * d d_l / d x
+            grad_x = torch.zeros(n, 3, dtype=torch.float32, device=dev)
+            for lvl in range(L + 1):
+                if (mask >> lvl) & 1:
+                    grad_x.addcmul_(g[lvl][:, None], net.sdf_grad_hip(x, lvl)[1])
+            grad_x = grad_x.to(ctx.x_dtype).reshape(ctx.x_shape)
+        if not any(ctx.needs_input_grad[4:]):
+            return (None, grad_x, None, None, *([None] * len(ctx.needs_input_grad[4:])))
         used_dec = {0} if net.num_decoder == 1 else {i for i in range(L + 1) if (mask >> i) & 1}
         G = _lib.NglodModelGrads()
         out = {}
@@ -76,7 +91,7 @@ class _SdfFunction(torch.autograd.Function):
                 grads.append(out[(kind, idx)])
             else:
                 grads.append(None)
-        return (None, None, None, None, *grads)
+        return (None, grad_x, None, None, *grads)
 
 
 def sdf_with_grad(net, x, lod, all_levels):

@@ -236,6 +236,41 @@ class OctreeSDF(nn.Module):
                    "nglod_sdf_fwd")
         return out
 
+    def gradient_hip(self, x, lod, method="autodiff", mask=None, normalize=False, form=_lib.FP32):
+        """d sdf(x, lod) / d x [N, 3] (diffutils.gradient, sdf-net/lib/diffutils.py:29-84)
+        by 'finitediff', 'tetrahedron' or 'autodiff' (the derivative torch
+        autograd computes), FP32.  mask: optional bool [N]; rows where it is
+        False are 0.  normalize: F.normalize(eps=1e-5) of each row, as
+        SphereTracer.forward applies to its normals.  No autograd graph."""
+        if method not in _lib.GRAD_METHODS:
+            raise NotImplementedError(f"gradient method {method!r}")
+        xf = x.detach().reshape(-1, 3).float().contiguous()
+        n = xf.shape[0]
+        out = torch.empty(n, 3, device=x.device, dtype=torch.float32)
+        mk = None
+        if mask is not None:
+            mk = mask.reshape(-1).to(device=x.device, dtype=torch.uint8).contiguous()
+            if mk.numel() != n:
+                raise ValueError("mask must hold one flag per point")
+        m = self.descriptor(form)
+        lib = _lib.load()
+        _lib.check(lib.nglod_gradient(m, _lib.ptr(xf), n, int(lod), form, _lib.GRAD_METHODS[method], _lib.ptr(mk),
+                                      1 if normalize else 0, _lib.ptr(out), _lib.stream_of(x)), "nglod_gradient")
+        return out
+
+    def sdf_grad_hip(self, x, lod):
+        """(sdf [N, 1], d sdf / d x [N, 3]) at `lod` in one pass, FP32 (the
+        autodiff gradient)."""
+        xf = x.detach().reshape(-1, 3).float().contiguous()
+        n = xf.shape[0]
+        y = torch.empty(n, 1, device=x.device, dtype=torch.float32)
+        g = torch.empty(n, 3, device=x.device, dtype=torch.float32)
+        m = self.descriptor(_lib.FP32)
+        lib = _lib.load()
+        _lib.check(lib.nglod_sdf_grad(m, _lib.ptr(xf), n, int(lod), _lib.FP32, _lib.ptr(y), _lib.ptr(g),
+                                      _lib.stream_of(x)), "nglod_sdf_grad")
+        return y, g
+
     def sdf_all_hip(self, x, lod):
         xf = x.detach().reshape(-1, 3).float().contiguous()
         n = xf.shape[0]

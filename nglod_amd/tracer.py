@@ -200,9 +200,8 @@ class SphereTracer(BaseTracer):
         in-loop SDF evaluation count."""
         if not hasattr(net, "descriptor"):
             raise TypeError("SphereTracer runs nglod_amd models (OctreeSDF); got " + type(net).__name__)
-        if self.grad_method != "finitediff":
-            raise NotImplementedError("the fused tracer computes finite-difference normals "
-                                      "(grad_method='finitediff', the reference default)")
+        if self.grad_method not in ("finitediff", "tetrahedron", "autodiff"):
+            raise NotImplementedError(f"grad_method {self.grad_method!r}")
         _lib.require_gpu(ray_o, ray_d)
         lod = net.lod if net.lod is not None else net.num_lods - 1
         o = ray_o.detach().float().contiguous()
@@ -220,7 +219,10 @@ class SphereTracer(BaseTracer):
         p.min_dis = float(self.min_dis)
         p.step_size = float(self.step_size)
         p.far_clip = float(self.camera_clamp[1])
-        p.normals = 1
+        # finite differences run inside the trace; the other methods after it,
+        # on the hits (SphereTracer.py:128-130)
+        fused_normals = self.grad_method == "finitediff"
+        p.normals = 1 if fused_normals else 0
         p.refine_steps = int(self.refine_steps)
         p.return_dist = float(self.return_dist)
         p.near_dist = float(self.near_dist)
@@ -232,6 +234,77 @@ class SphereTracer(BaseTracer):
         _lib.check(lib.nglod_trace_dense(m, _lib.ptr(o), _lib.ptr(d), n, lod, p, form, _lib.ptr(x), _lib.ptr(t),
                                          _lib.ptr(hit), _lib.ptr(normal), _lib.ptr(ws), ws.numel(), _lib.ptr(evals),
                                          _lib.ptr(ray_steps), _lib.stream_of(o)), "nglod_trace_dense")
+        if not fused_normals:
+            normal = net.gradient_hip(x, lod, self.grad_method, mask=hit, normalize=True, form=form)
         self.last_evals = evals
         self.last_pass_ms = list(pass_ms) if pass_ms is not None else None
         return RenderBuffer(x=x, depth=t, hit=hit, normal=normal)
+
+    def get_min(self, net, ray_o, ray_d):
+        """SphereTracer.get_min (SphereTracer.py:134-218), FP32 whatever the
+        form: the march of every ray from its box entry (from its origin when
+        it has none), keeping in min_x the point the reference's loop pairs
+        with the smallest distance seen; normals are the raw (unnormalised)
+        gradient by grad_method at the hits, zero elsewhere.  The batch loop's
+        stopping rule is kept, so a ray's min_x can depend on the batch, as in
+        the reference.  The reference builds its RenderBuffer with a `minx`
+        keyword the dataclass does not have (its field is min_x), which raises;
+        this returns min_x in that field."""
+        if not hasattr(net, "descriptor"):
+            raise TypeError("SphereTracer runs nglod_amd models (OctreeSDF); got " + type(net).__name__)
+        if self.grad_method not in ("finitediff", "tetrahedron", "autodiff"):
+            raise NotImplementedError(f"grad_method {self.grad_method!r}")
+        _lib.require_gpu(ray_o, ray_d)
+        lod = net.lod if net.lod is not None else net.num_lods - 1
+        o = ray_o.detach().float().contiguous()
+        d = ray_d.detach().float().contiguous()
+        n = o.shape[0]
+        dev = o.device
+        x = torch.empty(n, 3, device=dev, dtype=torch.float32)
+        t = torch.empty(n, 1, device=dev, dtype=torch.float32)
+        hit = torch.empty(n, device=dev, dtype=torch.bool)
+        min_x = torch.empty(n, 3, device=dev, dtype=torch.float32)
+        p = _lib.NglodTraceParams()
+        p.num_steps = int(self.num_steps)
+        p.min_dis = float(self.min_dis)
+        p.step_size = float(self.step_size)
+        p.far_clip = float(self.camera_clamp[1])
+        lib = _lib.load()
+        ws = torch.empty(max(int(lib.nglod_trace_min_workspace_size(n)), 256), dtype=torch.uint8, device=dev)
+        _lib.check(lib.nglod_trace_min(net.descriptor(_lib.FP32), _lib.ptr(o), _lib.ptr(d), n, lod, p, _lib.ptr(x),
+                                       _lib.ptr(t), _lib.ptr(hit), _lib.ptr(min_x), _lib.ptr(ws), ws.numel(),
+                                       _lib.stream_of(o)), "nglod_trace_min")
+        normal = net.gradient_hip(x, lod, self.grad_method, mask=hit, normalize=False)
+        return RenderBuffer(x=x, depth=t, hit=hit, normal=normal, min_x=min_x)
+
+    def sample_surface(self, n, net, seed=0, max_batches=1000):
+        """SphereTracer.sample_surface (SphereTracer.py:220-245): batches of n
+        rays with origins uniform in [-1, 1]^3 and directions uniform on the
+        sphere, traced until at least n hits are collected; returns every
+        collected hit point ([M, 3], M >= n).  The device is the model's (the
+        reference reads a self.device it never sets).  As in the reference,
+        the tracer reports an origin inside the box as a hit at depth 0 (its
+        box entry skips such rays, SphereTracer.py:53 with
+        sol_nglod_kernel.cu:109-112), so with these origins the points are
+        mostly the origins themselves; sol.voxel_sparsify starts its rays
+        outside the box for a sample of the surface.  Rays come from seeded
+        counter streams (nglod_sample_uniform / nglod_sample_directions)."""
+        dev = net.features[0].fm.device
+        lib = _lib.load()
+        pts, total = [], 0
+        for b in range(max_batches):
+            o = torch.empty(n, 3, device=dev, dtype=torch.float32)
+            d = torch.empty(n, 3, device=dev, dtype=torch.float32)
+            s = _lib.stream_of(o)
+            _lib.check(lib.nglod_sample_uniform(n, 2 * (seed * max_batches + b), _lib.ptr(o), s),
+                       "nglod_sample_uniform")
+            _lib.check(lib.nglod_sample_directions(n, 2 * (seed * max_batches + b) + 1, _lib.ptr(d), s),
+                       "nglod_sample_directions")
+            rb = self.forward(net, o, d)
+            pts.append(rb.x[rb.hit])
+            total += pts[-1].shape[0]
+            if total >= n:
+                break
+            if b == 49:
+                print("Taking an unusually long time to sample desired # of points.")
+        return torch.cat(pts, dim=0)

@@ -446,3 +446,101 @@ def cumulative_grids(model, prec="f32"):
             f32 = acc.astype(np.float32).reshape(-1)
             out.append((f32, f32.astype(np.float16)))
     return out
+
+
+def tetra_gradient(model, x, lod, prec="f32"):
+    """diffutils.gradient(..., 'tetrahedron') (sdf-net/lib/diffutils.py:38-60):
+    taps at x + h k_j, k_j the corners (1,-1,-1), (-1,-1,1), (-1,1,-1),
+    (1,1,1) of a tetrahedron, h = 1/192 in the working precision; gradient
+    (k_0 f_0 + k_1 f_1 + k_2 f_2 + k_3 f_3) / (4 h), summed left to right."""
+    dt = _dt(prec)
+    x = np.ascontiguousarray(x, dtype=dt).reshape(-1, 3)
+    h = dt(1.0 / (64.0 * 3.0))
+    ks = [np.array(k, dtype=dt) for k in ([1, -1, -1], [-1, -1, 1], [-1, 1, -1], [1, 1, 1])]
+    terms = [k[None, :] * sdf(model, x + (h * k)[None, :], lod, prec)[:, None] for k in ks]
+    return (((terms[0] + terms[1]) + terms[2]) + terms[3]) / dt((1.0 / (64.0 * 3.0)) * 4.0)
+
+
+def sdf_grad(model, x, lod):
+    """OctreeSDF.sdf(x, lod) (sdf-net/lib/models/OctreeSDF.py:94-155) and its
+    gradient with respect to x as torch autograd computes it, in FP64:
+    grid_sample's backward (align_corners=True, padding_mode='border': d u /
+    d x = r/2, zero where the unclamped u is <= 0 or >= r, trilinear weight
+    derivatives in the cell [floor(u), floor(u) + 1]), ReLU's (zero where the
+    pre-activation is <= 0) and the xyz columns of a position-dependent
+    decoder.  Returns (y [n], g [n, 3])."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    n = x.shape[0]
+    C = model.fdim
+    s = np.zeros((n, C))
+    levels = []
+    for l in range(lod + 1):
+        r = model.res[l]
+        g = model.grids[l].reshape(r + 1, r + 1, r + 1, C).astype(np.float64)
+        u_raw = ((x + 1.0) * 0.5) * r
+        u = np.clip(u_raw, 0.0, r)
+        i0 = np.minimum(np.floor(u), r - 1).astype(np.int64)
+        w = ((i0 + 1) - u, u - i0)
+        gate = np.where((u_raw > 0) & (u_raw < r), r * 0.5, 0.0)
+        corners = []
+        for k in range(8):
+            b = (k & 1, (k >> 1) & 1, k >> 2)
+            ck = g[i0[:, 2] + b[2], i0[:, 1] + b[1], i0[:, 0] + b[0]]
+            wk = w[b[0]][:, 0] * w[b[1]][:, 1] * w[b[2]][:, 2]
+            s = s + wk[:, None] * ck
+            corners.append((b, ck))
+        levels.append((w, gate, corners))
+    dec = 0 if len(model.w0) == 1 else lod
+    W0 = model.w0[dec].astype(np.float64)
+    b0 = model.b0[dec].astype(np.float64)
+    w1 = model.w1[dec].astype(np.float64)
+    e = s if model.pos_invariant else np.concatenate([x, s], axis=1)
+    pre = e @ W0.T + b0
+    y = np.maximum(pre, 0.0) @ w1 + float(model.b1[dec][0])
+    dy_de = (w1[None, :] * (pre > 0)) @ W0
+    off = 0 if model.pos_invariant else 3
+    v = dy_de[:, off:]
+    grad = np.zeros((n, 3)) if model.pos_invariant else dy_de[:, :3].copy()
+    for w, gate, corners in levels:
+        d = np.zeros((n, 3))
+        for b, ck in corners:
+            pk = (ck * v).sum(axis=1)
+            for a in range(3):
+                o1, o2 = [q for q in range(3) if q != a]
+                sign = 1.0 if b[a] else -1.0
+                d[:, a] += sign * pk * w[b[o1]][:, o1] * w[b[o2]][:, o2]
+        grad += gate * d
+    return y, grad
+
+
+def get_min(model, o, d, lod, num_steps=128, min_dis=3e-4, step_size=1.0, far_clip=10.0):
+    """SphereTracer.get_min (sdf-net/lib/tracer/SphereTracer.py:134-218) for a
+    batch of rays, FP32, the reference's batch loop restated with numpy masks
+    and the oracle's FP32 sdf and box entry: returns x, t, hit and min_x."""
+    f32 = np.float32
+    o = np.ascontiguousarray(o, dtype=f32).reshape(-1, 3)
+    d = np.ascontiguousarray(d, dtype=f32).reshape(-1, 3)
+    x, t, _ = aabb(o, d, "f32")
+    x = x.copy()
+    t = t.reshape(-1).astype(f32)
+    D = sdf(model, x, lod, "f32").astype(f32)
+    Dp = D.copy()
+    mind = D.copy()
+    minx = x.copy()
+    hit = np.zeros(o.shape[0], dtype=bool)
+    md, md3, far, step = f32(min_dis), f32(min_dis * 3.0), f32(far_clip), f32(step_size)
+    for _ in range(num_steps):
+        hit = np.abs(t) < far
+        cond = (np.abs(D) > md) & (np.abs((D + Dp) / f32(2.0)) > md3) & hit
+        if not cond.any():
+            break
+        x = np.where(cond[:, None], o + d * t[:, None], x).astype(f32)
+        new = D < mind
+        mind[new] = D[new]
+        minx[new] = x[new]
+        Dp = np.where(cond, D, Dp)
+        if cond.any():
+            D[cond] = sdf(model, x[cond], lod, "f32").astype(f32) * step
+        t = np.where(cond, t + D, t).astype(f32)
+    hit = hit & ~(np.abs(x) > 1.0).any(axis=1)
+    return dict(x=x, t=t, hit=hit, min_x=minx)
