@@ -26,13 +26,24 @@ FLAGS = [
 ]
 
 
+# Per-file additions.  mixed.hip (the mixed-form trace passes): MFMA results
+# in ordinary VGPRs.  Those kernels read every output element with the vector
+# ALU (ReLU, second layer); the AGPR form costs a v_accvgpr_read per element
+# and the AGPRs themselves (pass 1: 134 VGPRs instead of 152 + 16 AGPRs, 3-4%
+# faster on the 1280x720 frame; the batch-query kernels measured slower with
+# it and keep the default).
+EXTRA = {
+    "mixed.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+}
+
+
 def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
 def _compile(src):
     obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    cmd = [HIPCC, *FLAGS, *EXTRA.get(os.path.basename(src), []), "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")

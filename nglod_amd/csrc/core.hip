@@ -206,7 +206,8 @@ __global__ __launch_bounds__(kMarchBlock) void trace_dense_fp32_kernel(TraceArgs
     while (tail) {
         const uint64_t act = __ballot(s.phase != PH_IDLE);
         if (act == 0) break;
-        const float d = fp32_solo_step<POSINV>(a, &lds, &coarse, &solo_lv, sv, b1, lane, act, s.qx, s.qy, s.qz);
+        const float d = fp32_solo_step<POSINV>(a, &lds, &coarse, a.coarse, &solo_lv, sv, b1, lane, act, s.qx, s.qy,
+                                               s.qz);
         if (s.phase != PH_IDLE) {
             ++nevals;
             s.consume(a, d);

@@ -1,7 +1,6 @@
 // Mixed-precision form: FP16 grids, BF16 MFMA first decoder layer with FP32
 // accumulation, and FP32 re-evaluation near the surface.
 //   nglod_pack_mixed        FP32 model -> cumulative FP32 / FP16 grids + BF16 W0 fragments
-//   launch_sdf_mixed        nglod_sdf_fwd(form = NGLOD_MIXED)
 //   launch_trace_mixed      nglod_trace_dense(form = NGLOD_MIXED):
 //     pass 1  persistent mixed march.  When a ray stops on a surface test
 //             (|d| <= min_dis or the oscillation test) it rewinds its last
@@ -34,7 +33,16 @@ namespace nglod {
 // threads slower).  Capping registers for more waves per SIMD, or fewer,
 // measured slower for both.
 constexpr int kMixedBlock = 256;
-constexpr int kPass2Block = 512;
+// Pass 2 and the normals pass gather one FP32 cumulative grid per query and
+// keep nothing but the decoder in LDS.
+#ifndef NGLOD_MIXED_FP32_BLOCK
+#define NGLOD_MIXED_FP32_BLOCK 512
+#endif
+#ifndef NGLOD_MIXED_FP32_WAVES
+#define NGLOD_MIXED_FP32_WAVES 2
+#endif
+constexpr int kPass2Block = NGLOD_MIXED_FP32_BLOCK;
+constexpr int kMixedFp32Waves = NGLOD_MIXED_FP32_WAVES;
 // Pass 1 claims rays 64 ids at a time (the FP32 march claims per idle lane):
 // measured faster for the mixed march, slightly slower for the FP32 one.
 constexpr int kPass1Chunk = 64;
@@ -110,25 +118,6 @@ __global__ void pack_w0_kernel(const float* __restrict__ w0, int din, int off, u
 }
 
 // ---------------------------------------------------------------------------
-template <bool POSINV>
-__global__ __launch_bounds__(256) void sdf_mixed_kernel(ModelArgs m, int lod, int dec, const float* __restrict__ x,
-                                                        int64_t n, float* __restrict__ out,
-                                                        const uint32_t* __restrict__ order) {
-    __shared__ MixedLds lds;
-    stage_mixed_lds(m, dec, &lds);
-    __syncthreads();
-    const uint32_t lane = lane_id();
-    const float b1 = m.b1[dec][0];
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = t < n;
-    const int64_t i = valid ? (order ? (int64_t)order[t] : t) : 0;
-    const float px = valid ? x[3 * i] : 0.0f;
-    const float py = valid ? x[3 * i + 1] : 0.0f;
-    const float pz = valid ? x[3 * i + 2] : 0.0f;
-    if (__ballot(valid) == 0) return;  // whole wave past the end (no barrier follows)
-    const float d = sdf_mixed_wave<POSINV>(m, lod, &lds, b1, lane, valid, px, py, pz);
-    if (valid) out[i] = d;
-}
 
 // Pre-evaluation (t, D) of a ray's last four mixed-precision evaluations (a
 // shift register), and the rewind at a mixed surface stop.
@@ -293,13 +282,11 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
 // RETURNS: near misses may be handed back to a later pass 1 (not in the last
 // round); a separate instantiation, as for pass 1.
 template <bool POSINV, bool RETURNS>
-__global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArgs a) {
+__global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void trace_mixed_pass2_kernel(TraceArgs a) {
     __shared__ Fp32Lds lds;
-    __shared__ CoarseLds32 coarse;
     __shared__ __align__(16) float solo_sv[kPass2Block / 64][kH];
     __shared__ SoloLevels solo_lv;
     stage_fp32_lds(a.m, a.dec, &lds);
-    stage_coarse32(a.m, a.coarse, &coarse);
     stage_solo_levels(a.m, &solo_lv);
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
@@ -322,7 +309,7 @@ __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArg
             break;
         }
         const bool busy = s.phase != PH_IDLE;
-        const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz, a.coarse, &coarse);
+        const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
             s.consume(a, d);
@@ -335,7 +322,8 @@ __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArg
     while (tail) {
         const uint64_t act = __ballot(s.phase != PH_IDLE);
         if (act == 0) break;
-        const float d = fp32_solo_step<POSINV>(a, &lds, &coarse, &solo_lv, sv, b1, lane, act, s.qx, s.qy, s.qz);
+        const float d = fp32_solo_step<POSINV>(a, &lds, nullptr, CoarsePlan{0, 0}, &solo_lv, sv, b1, lane, act,
+                                               s.qx, s.qy, s.qz);
         if (s.phase != PH_IDLE) {
             ++nevals;
             s.consume(a, d);
@@ -350,11 +338,9 @@ __global__ __launch_bounds__(kPass2Block) void trace_mixed_pass2_kernel(TraceArg
 // point in x[ray]), one hit per lane, six evaluations each; waves claim
 // consecutive rays, so their hits are neighbours in the frame.
 template <bool POSINV>
-__global__ __launch_bounds__(kMarchBlock) void normals_pass_kernel(TraceArgs a) {
+__global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void normals_pass_kernel(TraceArgs a) {
     __shared__ Fp32Lds lds;
-    __shared__ CoarseLds32 coarse;
     stage_fp32_lds(a.m, a.dec, &lds);
-    stage_coarse32(a.m, a.coarse, &coarse);
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
@@ -378,10 +364,8 @@ __global__ __launch_bounds__(kMarchBlock) void normals_pass_kernel(TraceArgs a) 
             float pp[3] = {p[0] + 0.0f, p[1] + 0.0f, p[2] + 0.0f}, pm[3] = {p[0] - 0.0f, p[1] - 0.0f, p[2] - 0.0f};
             pp[ax] = p[ax] + kFdH;
             pm[ax] = p[ax] - kFdH;
-            const float fp =
-                sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pp[0], pp[1], pp[2], a.coarse, &coarse);
-            const float fm =
-                sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pm[0], pm[1], pm[2], a.coarse, &coarse);
+            const float fp = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pp[0], pp[1], pp[2]);
+            const float fm = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pm[0], pm[1], pm[2]);
             g[ax] = (fp - fm) / kFdTwoH;
         }
         if (valid) {
@@ -842,17 +826,6 @@ static int grid_for(K kernel, int64_t n, int block) {
     return (int)(want < cap ? (want < 1 ? 1 : want) : cap);
 }
 
-int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out,
-                     const uint32_t* order, hipStream_t s) {
-    const int64_t blocks = (n + 255) / 256;
-    if (m.pos_inv)
-        hipLaunchKernelGGL(sdf_mixed_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, m, lod, dec, x, n, out,
-                           order);
-    else
-        hipLaunchKernelGGL(sdf_mixed_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, m, lod, dec, x, n, out,
-                           order);
-    return check_launch("sdf_mixed_kernel");
-}
 
 __global__ void clear_words_kernel(uint32_t* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -875,7 +848,7 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
     const int g1 = grid_for(trace_mixed_pass1_kernel<POSINV, false>, n, kMixedBlock);
     const int g1r = grid_for(trace_mixed_pass1_kernel<POSINV, true>, n, kMixedBlock);
     const int g2 = grid_for(trace_mixed_pass2_kernel<POSINV, false>, n, kPass2Block);
-    const int g3 = grid_for(normals_pass_kernel<POSINV>, n, kMarchBlock);
+    const int g3 = grid_for(normals_pass_kernel<POSINV>, n, kPass2Block);
     const int words = (int)((n + 3) / 4);
     const dim3 cg((unsigned)((words + 255) / 256));
     const int rounds = a.return_dist > 0.0f ? kMixedRounds : 1;
@@ -907,7 +880,7 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
         timer.mark(kPassFp32);
     }
     if (a.normals) {
-        hipLaunchKernelGGL(normals_pass_kernel<POSINV>, dim3(g3), dim3(kMarchBlock), 0, s, a);
+        hipLaunchKernelGGL(normals_pass_kernel<POSINV>, dim3(g3), dim3(kPass2Block), 0, s, a);
         int rc = check_launch("normals_pass_kernel");
         if (rc) return rc;
         timer.mark(kPassNormals);

@@ -101,17 +101,33 @@ __device__ __forceinline__ uint32_t pack_bf16(__half2 h) {
     return __builtin_bit_cast(uint32_t, v);
 }
 
-// Second layer over one 32x32 output tile held in registers: sum of
-// w1[row] * relu(C + b0[row] + wxyz[row] . xyz) over this lane's 16 rows.
+// b0 of this lane's 16 rows of hidden tile h (rows 32h + 8g + 4 half + 0..3,
+// g = 0..3): the first MFMA of the tile accumulates onto it, so the bias
+// costs no vector instruction.
+__device__ __forceinline__ f32x16 bias_tile(int h, uint32_t half, const MixedLds* s) {
+    f32x16 c;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const float4 b = *reinterpret_cast<const float4*>(&s->b0[32 * h + 8 * g + 4 * (int)half]);
+        c[4 * g + 0] = b.x;
+        c[4 * g + 1] = b.y;
+        c[4 * g + 2] = b.z;
+        c[4 * g + 3] = b.w;
+    }
+    return c;
+}
+
+// Second layer over one 32x32 output tile held in registers (the first
+// layer's output with b0 already in it): sum of
+// w1[row] * relu(C + wxyz[row] . xyz) over this lane's 16 rows.
 __device__ __forceinline__ float layer2_partial(const f32x16& c, int h, uint32_t half, const MixedLds* s, bool xyz,
                                                 float px, float py, float pz) {
     float acc = 0.0f;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
         const int row0 = 32 * h + 8 * g + 4 * (int)half;
-        const float4 b = *reinterpret_cast<const float4*>(&s->b0[row0]);
         const float4 w = *reinterpret_cast<const float4*>(&s->w1[row0]);
-        float v0 = c[4 * g + 0] + b.x, v1 = c[4 * g + 1] + b.y, v2 = c[4 * g + 2] + b.z, v3 = c[4 * g + 3] + b.w;
+        float v0 = c[4 * g + 0], v1 = c[4 * g + 1], v2 = c[4 * g + 2], v3 = c[4 * g + 3];
         if (xyz) {
             const float4 q0 = *reinterpret_cast<const float4*>(s->wxyz[row0 + 0]);
             const float4 q1 = *reinterpret_cast<const float4*>(s->wxyz[row0 + 1]);
@@ -181,14 +197,12 @@ __device__ __forceinline__ float decode_mixed_wave(const __half2 (&acc)[kC / 2],
         const bf16x8 a0 = __builtin_bit_cast(bf16x8, s->wfrag[(h * 2 + 0) * 64 + lane]);
         const bf16x8 a1 = __builtin_bit_cast(bf16x8, s->wfrag[(h * 2 + 1) * 64 + lane]);
         if (use0) {
-            f32x16 c0 = {};
-            c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B0[0], c0, 0, 0, 0);
+            f32x16 c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B0[0], bias_tile(h, half, s), 0, 0, 0);
             c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, B0[1], c0, 0, 0, 0);
             part0 += layer2_partial(c0, h, half, s, !POSINV, qx0, qy0, qz0);
         }
         if (use1) {
-            f32x16 c1 = {};
-            c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B1[0], c1, 0, 0, 0);
+            f32x16 c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B1[0], bias_tile(h, half, s), 0, 0, 0);
             c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, B1[1], c1, 0, 0, 0);
             part1 += layer2_partial(c1, h, half, s, !POSINV, qx1, qy1, qz1);
         }

@@ -348,8 +348,8 @@ __device__ __forceinline__ int64_t claim_hits(uint32_t* counter, const uint8_t* 
 // made the pass 25% slower).
 template <bool POSINV>
 __device__ __forceinline__ float fp32_solo_step(const TraceArgs& a, const Fp32Lds* lds, const CoarseLds32* coarse,
-                                                const SoloLevels* lv, float* sv, float b1, uint32_t lane,
-                                                uint64_t act, float qx, float qy, float qz) {
+                                                CoarsePlan cp, const SoloLevels* lv, float* sv, float b1,
+                                                uint32_t lane, uint64_t act, float qx, float qy, float qz) {
     float d = 0.0f;
     uint64_t mm = act;
     while (mm) {
@@ -358,7 +358,7 @@ __device__ __forceinline__ float fp32_solo_step(const TraceArgs& a, const Fp32Ld
         // broadcast through a lane shuffle: the address arithmetic of the
         // solo gather then stays in vector registers
         const float px = __shfl(qx, r), py = __shfl(qy, r), pz = __shfl(qz, r);
-        const float dr = sdf_fp32_solo<POSINV>(lv, a.lod, lds, b1, lane, sv, px, py, pz, a.coarse, coarse);
+        const float dr = sdf_fp32_solo<POSINV>(lv, a.lod, lds, b1, lane, sv, px, py, pz, cp, coarse);
         if ((int)lane == r) d = dr;
     }
     return d;
