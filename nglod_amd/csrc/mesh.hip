@@ -43,8 +43,19 @@ struct TriPrep {
     double idet[13];             // 0 where |det| < 1e-8 (no intersection)
 };
 
-__device__ __forceinline__ double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+// The FP64 distance and stabbing arithmetic lets the compiler fuse a multiply
+// and an add (one rounding instead of two): the library is otherwise built
+// without contraction so that FP32 results follow the reference op by op, but
+// here the bar is the distance to 1e-12 relative (1e-15 absolute) and exact
+// signs, and the fused form issues about a quarter fewer FP64 instructions.
+#define NGLOD_FP64_CONTRACT _Pragma("clang fp contract(fast)")
+
+__device__ __forceinline__ double dot3(const double* a, const double* b) {
+    NGLOD_FP64_CONTRACT
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
 __device__ __forceinline__ void cross3(const double* a, const double* b, double* r) {
+    NGLOD_FP64_CONTRACT
     r[0] = a[1] * b[2] - a[2] * b[1];
     r[1] = a[2] * b[0] - a[0] * b[2];
     r[2] = a[0] * b[1] - a[1] * b[0];
@@ -85,6 +96,7 @@ __global__ void tri_prep_kernel(const float* __restrict__ mesh, int64_t nf, TriP
 
 __device__ __forceinline__ double clamp01(double x) { return fmax(0.0, fmin(x, 1.0)); }
 __device__ __forceinline__ double d2axmb(const double* a, double x, const double* b) {
+    NGLOD_FP64_CONTRACT
     const double t0 = a[0] * x - b[0], t1 = a[1] * x - b[1], t2 = a[2] * x - b[2];
     return t0 * t0 + t1 * t1 + t2 * t2;
 }
@@ -94,6 +106,7 @@ constexpr int kTriTile = 24;  // triangles per LDS tile (24 x 640 B = 15 KB)
 __global__ __launch_bounds__(256) void mesh2sdf_kernel(const float* __restrict__ pts, int64_t n,
                                                        const TriPrep* __restrict__ tris, int64_t nf,
                                                        float* __restrict__ dist, double* __restrict__ dist64) {
+    NGLOD_FP64_CONTRACT
     __shared__ TriPrep tile[kTriTile];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
