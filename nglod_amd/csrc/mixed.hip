@@ -26,15 +26,13 @@
 
 namespace nglod {
 
-// Pass 1 reads every level from global memory: with LDS copies of the coarse
-// levels (FP16 or FP32 layout) it measured 10-20% slower on the 1280x720 LOD-4
-// frame.  Pass 2 copies LODs 0-1 to LDS like the FP32 march, in 512-thread
-// blocks (3-5% faster than 256-thread blocks without them; 768 or 1024
-// threads slower).  Capping registers for more waves per SIMD, or fewer,
-// measured slower for both.
+// Pass 1 reads its grid from global memory: with LDS copies of coarse levels
+// (when it gathered every level) it measured 10-20% slower on the 1280x720
+// LOD-4 frame.  Pass 2 and the normals pass gather one FP32 cumulative grid
+// per query and keep only the decoder in LDS (pass 2 132 VGPRs, normals 114);
+// 512-thread blocks measured faster than 256 for pass 2 (0.51 against 0.59 ms),
+// and more waves per SIMD did not help either pass.
 constexpr int kMixedBlock = 256;
-// Pass 2 and the normals pass gather one FP32 cumulative grid per query and
-// keep nothing but the decoder in LDS.
 #ifndef NGLOD_MIXED_FP32_BLOCK
 #define NGLOD_MIXED_FP32_BLOCK 512
 #endif
