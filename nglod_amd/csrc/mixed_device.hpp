@@ -117,6 +117,15 @@ __device__ __forceinline__ f32x16 bias_tile(int h, uint32_t half, const MixedLds
     return c;
 }
 
+// ReLU as an integer max on the float's bits: the sign bit makes every
+// negative value (and -0) a negative integer, so max(bits, 0) keeps positive
+// values and gives +0 otherwise.  One v_max_i32; fmaxf(v, 0) on an MFMA
+// output costs a canonicalising v_max_f32 before the max itself.
+__device__ __forceinline__ float relu_bits(float v) {
+    const int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, b > 0 ? b : 0);
+}
+
 // Second layer over one 32x32 output tile held in registers (the first
 // layer's output with b0 already in it): sum of
 // w1[row] * relu(C + wxyz[row] . xyz) over this lane's 16 rows.
@@ -138,10 +147,10 @@ __device__ __forceinline__ float layer2_partial(const f32x16& c, int h, uint32_t
             v2 = __builtin_fmaf(q2.x, px, __builtin_fmaf(q2.y, py, __builtin_fmaf(q2.z, pz, v2)));
             v3 = __builtin_fmaf(q3.x, px, __builtin_fmaf(q3.y, py, __builtin_fmaf(q3.z, pz, v3)));
         }
-        acc = __builtin_fmaf(w.x, fmaxf(v0, 0.0f), acc);
-        acc = __builtin_fmaf(w.y, fmaxf(v1, 0.0f), acc);
-        acc = __builtin_fmaf(w.z, fmaxf(v2, 0.0f), acc);
-        acc = __builtin_fmaf(w.w, fmaxf(v3, 0.0f), acc);
+        acc = __builtin_fmaf(w.x, relu_bits(v0), acc);
+        acc = __builtin_fmaf(w.y, relu_bits(v1), acc);
+        acc = __builtin_fmaf(w.z, relu_bits(v2), acc);
+        acc = __builtin_fmaf(w.w, relu_bits(v3), acc);
     }
     return acc;
 }
