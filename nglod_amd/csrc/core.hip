@@ -116,35 +116,52 @@ int launch_sdf_fp32(const ModelArgs& m, int lod, int dec, const float* x, int64_
     return check_launch("sdf_fp32_kernel");
 }
 
+// Every level's prediction (the training forward): the running FP32 feature
+// sum per lane, and after each level the FP32 MFMA decoder of that level on
+// the wave's 64 queries (decode_fp32_wave, the trace's decoder), each level's
+// decoder staged in LDS in turn.  Every thread of the block runs every level
+// (no early exit: the staging barriers are block-wide).
 template <bool POSINV>
 __global__ __launch_bounds__(256) void sdf_all_fp32_kernel(ModelArgs m, int lod, const float* __restrict__ x,
                                                            int64_t n, float* __restrict__ out) {
+    __shared__ Fp32Lds lds;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float px = x[3 * i], py = x[3 * i + 1], pz = x[3 * i + 2];
+    const bool valid = i < n;
+    const uint32_t lane = lane_id();
+    const float px = valid ? x[3 * i] : 0.0f, py = valid ? x[3 * i + 1] : 0.0f, pz = valid ? x[3 * i + 2] : 0.0f;
     float f[kC];
 #pragma unroll
     for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+    int staged = -1;
     for (int l = 0; l <= lod; ++l) {
-        const int r = m.res[l];
-        const int S = r + 1;
-        const AxisW ax = axis_weights(px, r);
-        const AxisW ay = axis_weights(py, r);
-        const AxisW az = axis_weights(pz, r);
-        const float* base = m.grid[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
-        const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
-        const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
-        const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
-        corner_fma(f, base, wxy00 * az.w0);
-        corner_fma(f, base + sx, wxy10 * az.w0);
-        corner_fma(f, base + sy, wxy01 * az.w0);
-        corner_fma(f, base + sy + sx, wxy11 * az.w0);
-        corner_fma(f, base + sz, wxy00 * az.w1);
-        corner_fma(f, base + sz + sx, wxy10 * az.w1);
-        corner_fma(f, base + sz + sy, wxy01 * az.w1);
-        corner_fma(f, base + sz + sy + sx, wxy11 * az.w1);
         const int dec = m.num_dec == 1 ? 0 : l;
-        out[(int64_t)l * n + i] = decode_fp32<POSINV>(m, dec, px, py, pz, f);
+        if (dec != staged) {  // block-uniform
+            __syncthreads();
+            stage_fp32_lds(m, dec, &lds);
+            __syncthreads();
+            staged = dec;
+        }
+        if (valid) {
+            const int r = m.res[l];
+            const int S = r + 1;
+            const AxisW ax = axis_weights(px, r);
+            const AxisW ay = axis_weights(py, r);
+            const AxisW az = axis_weights(pz, r);
+            const float* base = m.grid[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
+            const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
+            const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
+            const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
+            corner_fma(f, base, wxy00 * az.w0);
+            corner_fma(f, base + sx, wxy10 * az.w0);
+            corner_fma(f, base + sy, wxy01 * az.w0);
+            corner_fma(f, base + sy + sx, wxy11 * az.w0);
+            corner_fma(f, base + sz, wxy00 * az.w1);
+            corner_fma(f, base + sz + sx, wxy10 * az.w1);
+            corner_fma(f, base + sz + sy, wxy01 * az.w1);
+            corner_fma(f, base + sz + sy + sx, wxy11 * az.w1);
+        }
+        const float d = decode_fp32_wave<POSINV>(f, &lds, m.b1[dec][0], lane, valid, px, py, pz);
+        if (valid) out[(int64_t)l * n + i] = d;
     }
 }
 

@@ -434,6 +434,36 @@ __global__ void bwd_run_final_kernel(const uint32_t* __restrict__ ukeys, const u
     grad[(size_t)ukeys[r] * kC + c] = s;
 }
 
+// The same sums for levels whose runs hold many pieces (the coarse grids:
+// 2^19 queries put ~500 pieces in each of LOD 0's 125 corners): one block of
+// kRunGroups x kC threads per run; thread (g, c) sums pieces g, g + kRunGroups,
+// ... of channel c in order (a row of kC partials is one coalesced load for
+// the block's 32 channels), then thread (0, c) adds the kRunGroups group sums
+// in group order.  A fixed order, so the result is the same every run.
+constexpr int kRunGroups = 8;
+
+__global__ __launch_bounds__(kRunGroups * kC) void bwd_run_final_block_kernel(
+    const uint32_t* __restrict__ ukeys, const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ piece_start,
+    const uint32_t* __restrict__ nruns, const float* __restrict__ partial, float* __restrict__ grad) {
+    __shared__ float gs[kRunGroups][kC];
+    const uint32_t r = blockIdx.x;
+    if (r >= *nruns) return;  // block-uniform
+    const int c = threadIdx.x % kC;
+    const int g = threadIdx.x / kC;
+    const uint32_t np = (cnt[r] + kRunPiece - 1) / kRunPiece;
+    const size_t p0 = piece_start[r];
+    float s = 0.0f;
+    for (uint32_t i = g; i < np; i += kRunGroups) s += partial[(p0 + i) * kC + c];
+    gs[g][c] = s;
+    __syncthreads();
+    if (g == 0) {
+        float t = gs[0][c];
+#pragma unroll
+        for (int k = 1; k < kRunGroups; ++k) t += gs[k][c];
+        grad[(size_t)ukeys[r] * kC + c] = t;
+    }
+}
+
 __global__ void fill_f32_kernel(float* p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0f;
@@ -668,10 +698,19 @@ int nglod_sdf_bwd(const nglod_model_t* model, const float* x, int64_t n, int32_t
                            cnts, pstart, nruns, L.max_pieces, ppart);
         rc = check_launch("bwd_piece_sum_kernel");
         if (rc) return rc;
-        const uint64_t fin = (uint64_t)L.max_runs * kC;
-        hipLaunchKernelGGL(bwd_run_final_kernel, dim3((unsigned)((fin + 255) / 256)), dim3(256), 0, s, ukeys, cnts, pstart,
-                           nruns, L.max_runs, ppart, grads->grid[j]);
-        rc = check_launch("bwd_run_final_kernel");
+        // levels averaging 8 or more pieces per corner: a block per run
+        const int64_t corners = (int64_t)(m.res[j] + 1) * (m.res[j] + 1) * (m.res[j] + 1);
+        if (mm >= corners * 8 * kRunPiece) {
+            const uint32_t runs = (uint32_t)(corners < (int64_t)L.max_runs ? corners : (int64_t)L.max_runs);
+            hipLaunchKernelGGL(bwd_run_final_block_kernel, dim3(runs), dim3(kRunGroups * kC), 0, s, ukeys, cnts, pstart,
+                               nruns, ppart, grads->grid[j]);
+            rc = check_launch("bwd_run_final_block_kernel");
+        } else {
+            const uint64_t fin = (uint64_t)L.max_runs * kC;
+            hipLaunchKernelGGL(bwd_run_final_kernel, dim3((unsigned)((fin + 255) / 256)), dim3(256), 0, s, ukeys, cnts,
+                               pstart, nruns, L.max_runs, ppart, grads->grid[j]);
+            rc = check_launch("bwd_run_final_kernel");
+        }
         if (rc) return rc;
     }
     return NGLOD_OK;
