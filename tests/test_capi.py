@@ -98,3 +98,42 @@ def test_shading_argument_checks():
     assert lib.nglod_matcap_shade(16, 16, 16, 8, None, 16, 8, 8, 2, None, None, 16, None) == _lib.ERR_ARG  # 2 ch
     assert lib.nglod_matcap_shade(None, 16, 16, 8, None, 16, 8, 8, 3, None, None, 16, None) == _lib.ERR_ARG
     assert lib.nglod_matcap_shade(None, None, None, 0, None, None, 8, 8, 3, None, None, None, None) == _lib.OK
+
+
+def test_gradient_and_pack_argument_checks():
+    """nglod_gradient, nglod_sdf_grad, nglod_trace_min and nglod_pack_mixed
+    reject bad arguments before any launch (fake device pointers are never
+    dereferenced)."""
+    lib = _lib.load()
+    m = _lib.NglodModel()
+    m.num_lods, m.feature_dim, m.hidden_dim, m.num_decoders = 3, 32, 128, 1
+    for i in range(3):
+        m.res[i] = 4 << i
+        m.grid[i] = 4096 * (i + 1)
+    m.w0[0] = m.b0[0] = m.w1[0] = m.b1[0] = 4096
+    assert lib.nglod_gradient(ctypes.byref(m), 16, 8, 2, 0, 7, None, 0, 16, None) == _lib.ERR_ARG  # method
+    assert lib.nglod_gradient(ctypes.byref(m), 16, 8, 5, 0, 2, None, 0, 16, None) == _lib.ERR_ARG  # lod
+    assert lib.nglod_gradient(ctypes.byref(m), None, 8, 2, 0, 2, None, 0, 16, None) == _lib.ERR_ARG  # NULL x
+    assert lib.nglod_gradient(ctypes.byref(m), None, 0, 2, 0, 2, None, 0, None, None) == _lib.OK  # empty
+    # the mixed form needs the cumulative grids nglod_pack_mixed builds
+    assert lib.nglod_gradient(ctypes.byref(m), 16, 8, 2, 1, 2, None, 0, 16, None) == _lib.ERR_ARG
+    assert b"cumulative" in lib.nglod_last_error()
+    assert lib.nglod_sdf_grad(ctypes.byref(m), 16, 8, 2, 0, None, None, None) == _lib.ERR_ARG  # NULL grad
+    p = _lib.NglodTraceParams()
+    p.num_steps = 0
+    assert lib.nglod_trace_min(ctypes.byref(m), 16, 16, 8, 2, ctypes.byref(p), 16, 16, 16, 16, 16, 4096,
+                               None) == _lib.ERR_ARG
+    p.num_steps = 8
+    need = lib.nglod_trace_min_workspace_size(8)
+    assert need >= 8 * 4
+    assert lib.nglod_trace_min(ctypes.byref(m), 16, 16, 8, 2, ctypes.byref(p), 16, 16, 16, 16, 16, need - 1,
+                               None) == _lib.ERR_WORKSPACE
+    # resolutions that do not nest (3 does not divide 4) are refused before any launch
+    for i in range(3):
+        m.grid_cum[i] = 8192 * (i + 1)
+        m.grid_h[i] = 8192 * (i + 1)
+    m.w0_bf16[0] = 8192
+    m.res[0], m.res[1], m.res[2] = 3, 4, 8
+    assert lib.nglod_pack_mixed(ctypes.byref(m), None) == _lib.ERR_UNSUPPORTED
+    m.res[0] = 0
+    assert lib.nglod_pack_mixed(ctypes.byref(m), None) == _lib.ERR_ARG
