@@ -133,3 +133,49 @@ def test_get_min_march_equals_the_trace():
     off = got["min_x"] - o
     cross = np.cross(off.astype(np.float64), d.astype(np.float64))
     assert np.abs(cross).max() < 1e-4 * max(1.0, np.abs(off).max())
+
+
+def _plane_model(normal, offset):
+    """A model whose SDF is exactly affine, d(x) = normal . x + offset: an
+    affine feature field (channel 0 = normal . x), identity first layer with
+    biases that keep every unit active, second layer reading unit 0."""
+    net = small_model(num_lods=1, seed=42)
+    c = np.linspace(-1.0, 1.0, 5)
+    z, y, xg = np.meshgrid(c, c, c, indexing="ij")
+    with torch.no_grad():
+        fm = net.features[0].fm
+        fm.zero_()
+        fm[0, 0].copy_(torch.from_numpy(normal[0] * xg + normal[1] * y + normal[2] * z))
+        l = net.louts[0]
+        W0 = np.zeros((128, 32))
+        W0[np.arange(32), np.arange(32)] = 1.0
+        l[0].weight.copy_(torch.from_numpy(W0))
+        l[0].bias.fill_(8.0)  # every unit active
+        l[2].weight.zero_()
+        l[2].weight[0, 0] = 1.0
+        l[2].bias.fill_(offset - 8.0)
+    return net
+
+
+def test_get_min_on_a_plane_tracks_the_decreasing_distance():
+    """Rays approaching the plane x + 0.5 = 0 see the distance fall at every
+    step, so the smallest distance is the last one evaluated and get_min's
+    min_x is the final point (a comparison that never moves min_x, or that
+    keeps the point where the distance was evaluated, gives the previous one);
+    rays leaving the plane keep their first point."""
+    m = arrays(_plane_model(np.array([1.0, 0.0, 0.0]), 0.5))
+    rng = np.random.default_rng(43)
+    n = 200
+    o = np.stack([np.full(n, 2.5), rng.uniform(-0.4, 0.4, n), rng.uniform(-0.4, 0.4, n)], 1).astype(np.float32)
+    d = np.stack([-np.ones(n), rng.uniform(-0.15, 0.15, n), rng.uniform(-0.15, 0.15, n)], 1)
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    got = oracle.get_min(m, o, d, 0, num_steps=64)
+    assert got["hit"].all()
+    np.testing.assert_allclose(got["x"][:, 0], -0.5, atol=1e-3)
+    assert np.array_equal(got["min_x"], got["x"])
+    # moving away from the plane from inside the box: the first point stays
+    o2 = np.stack([np.full(n, -0.2), rng.uniform(-0.4, 0.4, n), rng.uniform(-0.4, 0.4, n)], 1).astype(np.float32)
+    d2 = -d
+    away = oracle.get_min(m, o2, d2, 0, num_steps=64)
+    ent, _, _ = oracle.aabb(o2, d2)
+    assert np.array_equal(away["min_x"], ent)
