@@ -519,11 +519,11 @@ def main():
         if form == "fp32":
             passes = {"trace_dense_fp32_kernel": (pass_ms[0], ev[0] * b32 + n * (24 + 29))}
         else:
-            n_fp32 = max(ev[0] - 6 * hits, 0)
+            # pass 2 makes every FP32 evaluation: the continuation and the six
+            # normal taps of each hit (it writes the hits' normals, 13 B each)
             passes = {
                 "trace_mixed_pass1_kernel": (pass_ms[0], ev[1] * b16 + n * 24),
-                "trace_mixed_pass2_kernel": (pass_ms[1], n_fp32 * b32),
-                "normals_pass_kernel": (pass_ms[2], 6 * hits * b32 + hits * 13),
+                "trace_mixed_pass2_kernel": (pass_ms[1], ev[0] * b32 + hits * 13),
             }
         # decoder FLOPs (2*H*D_in + 2*H per evaluation) over the frame's kernel
         # time: the MFMA utilisation figure (FP32 MFMA peak 157.3, BF16 2500 TFLOP/s)

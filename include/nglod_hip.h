@@ -92,8 +92,9 @@ typedef struct nglod_trace_params {
                              waits for the last one (the call becomes
                              synchronous) and writes milliseconds per pass:
                              [0] the march kernel (FP32 form) or the mixed
-                             march, [1] the FP32 continuation (mixed form),
-                             [2] the normals pass (mixed form), [3] counter
+                             march, [1] the FP32 continuation (mixed form;
+                             it also takes the normals of the form's hits),
+                             [2] 0 (no separate normals pass), [3] counter
                              resets.  For measurement, not for serving. */
 } nglod_trace_params_t;
 
@@ -227,8 +228,8 @@ size_t nglod_trace_pool_workspace_size(int64_t pool_rays);
 /* nglod_trace_dense (NGLOD_MIXED, SphereTracer semantics) for up to 64
  * requests at once, each with its own model (an index into model_table;
  * `models` is the same host array, for validation) and LOD: one launch of the
- * persistent mixed march over all their rays, one FP32 continuation pass and
- * one normals pass.  ray_o, ray_d, x, t, hit, normal are the pool's per-slot
+ * persistent mixed march over all their rays and one FP32 continuation pass
+ * (which also takes the hits' normals).  ray_o, ray_d, x, t, hit, normal are the pool's per-slot
  * arrays ([pool_rays][3] or [pool_rays]); each request's outputs land in its
  * own slots and are bit-identical to a nglod_trace_dense call on those rays
  * alone (the FP32 continuation pass here runs without the single-request

@@ -2,8 +2,8 @@
 // workgroup in LDS, so that the gather of those levels costs LDS reads instead
 // of vector-memory (TA/TD) work.  At the standard resolutions LODs 0 and 1
 // (r = 4, 8) hold 125 + 729 = 854 corners: 109 KB.  Used by the FP32 march and
-// the normals pass, where it measured faster; the mixed march gathers every
-// level from global memory.
+// the AO shading pass, where it measured faster; the mixed form's passes
+// gather one cumulative grid from global memory.
 //
 // Layout: corner `idx` (level 0 first, then level 1) is 8 16-byte chunks;
 // logical chunk k is stored at chunk k ^ (idx & 7), which spreads the lanes of

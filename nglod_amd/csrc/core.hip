@@ -310,7 +310,7 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
                      void* stream) {
     // Normals are FP32 in both forms (finite differences of h = 1/192 need
     // more than BF16 resolution); the mixed form takes them from the FP32
-    // cumulative grid, as its trace's normals pass does.
+    // cumulative grid, as its trace's FP32 pass does.
     ModelArgs m;
     int rc = make_model_args(model, lod, form, &m);
     if (rc) return rc;
@@ -329,11 +329,10 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
 }
 
 // Workspace: kWsCounters bytes of counters (the ray queues, then the claim
-// counters of pass 2, of a pass 1 over returned rays and of the normals pass,
-// kQueues each); for the mixed form then hand-off records [n], returned records [n], hand-off flags
+// counters of pass 2 and of a pass 1 over returned rays, kQueues each); for the mixed form then hand-off records [n], returned records [n], hand-off flags
 // [n] and return flags [n] (each flag array padded to 256 bytes).
-static constexpr size_t kWsCounters = (16 * kQueues + 255) / 256 * 256;
-static constexpr int kCounterWords = 4 * kQueues;
+static constexpr int kCounterWords = 3 * kQueues;
+static constexpr size_t kWsCounters = (4 * kCounterWords + 255) / 256 * 256;
 
 static size_t flag_bytes(int64_t n) { return ((size_t)n + 255) / 256 * 256; }
 
@@ -397,15 +396,14 @@ static int trace_dense_run(const nglod_model_t* model, const ModelArgs& m, const
     a.queue = w.counters;
     a.handoff_queue = a.queue + kQueues;
     a.ret_queue = a.queue + 2 * kQueues;
-    a.normal_queue = a.queue + 3 * kQueues;
     a.handoff = w.handoff;
     a.ret = w.ret;
     a.hflag = w.hflag;
     a.rflag = w.rflag;
     a.return_dist = (float)params->return_dist;
-    // The FP32 form takes its normal taps in-lane: a separate normals pass
-    // measured 3-4% slower for it.
-    a.defer_normals = form == NGLOD_MIXED ? 1 : 0;
+    // In-lane normal taps; the mixed passes set their own.  A separate
+    // normals pass measured 3-4% slower for the FP32 form.
+    a.defer_normals = 0;
     a.solo_lanes = solo_lanes_setting();
     a.coarse = coarse_plan(m, a.lod);
     a.replay_steps = params->refine_steps < 1 ? 1 : params->refine_steps;

@@ -6,16 +6,17 @@
 //             (|d| <= min_dis or the oscillation test) it rewinds its last
 //             near-surface evaluations (at most replay_steps, back to the
 //             last step of at least near_dist) and writes its hand-off
-//             record at handoff[ray] (flag hflag[ray]); finished hits are
-//             written out without their normal.
+//             record at handoff[ray] (flag hflag[ray]); a ray it finishes as
+//             a hit is handed over the same way, in phase PH_N0.
 //     pass 2  persistent FP32 kernel over the hand-off records, claimed in
 //             ray order: re-evaluates the rewound steps and goes on in FP32
 //             to its FP32 stop; a ray whose FP32 step grows past return_dist
 //             (the mixed stop was a near miss) is handed back (ret[ray],
 //             rflag[ray]) and pass 1 resumes it.  Passes 1 and 2 run
 //             kMixedRounds times (once when return_dist = 0); the last pass 2
-//             keeps every ray.
-//     pass 3  FP32 normals of every hit, one hit per lane.
+//             keeps every ray.  Pass 2 takes the six FP32 normal taps of
+//             every hit, in-lane after its continuation or from a PH_N0
+//             record (there is no separate normals pass).
 // No pass appends to a shared list: the records are indexed by ray, so the
 // only atomics are the claim counters (one per wave per refill).  A single
 // append counter hit by every wave serialised the march kernels measurably.
@@ -28,10 +29,10 @@ namespace nglod {
 
 // Pass 1 reads its grid from global memory: with LDS copies of coarse levels
 // (when it gathered every level) it measured 10-20% slower on the 1280x720
-// LOD-4 frame.  Pass 2 and the normals pass gather one FP32 cumulative grid
-// per query and keep only the decoder in LDS (pass 2 132 VGPRs, normals 114);
-// 512-thread blocks measured faster than 256 for pass 2 (0.51 against 0.59 ms),
-// and more waves per SIMD did not help either pass.
+// LOD-4 frame.  Pass 2 gathers one FP32 cumulative grid per query and keeps
+// only the decoder in LDS (134 VGPRs); 512-thread blocks measured faster than
+// 256 for it (0.51 against 0.59 ms, before it took the normal taps), and more
+// waves per SIMD did not help.
 constexpr int kMixedBlock = 256;
 #ifndef NGLOD_MIXED_FP32_BLOCK
 #define NGLOD_MIXED_FP32_BLOCK 512
@@ -332,47 +333,6 @@ __global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void trace_mixed_pass
     add_evals(a, 0, nevals, lane);
 }
 
-// Pass 3: FP32 central-difference normals of the hits (hit[ray] = 1, the hit
-// point in x[ray]), one hit per lane, six evaluations each; waves claim
-// consecutive rays, so their hits are neighbours in the frame.
-template <bool POSINV>
-__global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void normals_pass_kernel(TraceArgs a) {
-    __shared__ Fp32Lds lds;
-    stage_fp32_lds(a.m, a.dec, &lds);
-    __syncthreads();
-    const float b1 = a.m.b1[a.dec][0];
-    const uint32_t lane = lane_id();
-    const uint32_t total = (uint32_t)a.nrays;
-    bool drained = false;
-    uint32_t nevals = 0;
-    for (;;) {
-        // uniform work per hit: one range in frame order, a claim per idle lane
-        const int64_t id = claim_hits(a.normal_queue, a.hit, total, lane, drained);
-        const bool valid = id >= 0;
-        if (__ballot(valid) == 0) break;
-        float p[3] = {0.0f, 0.0f, 0.0f};
-        if (valid) {
-            p[0] = a.x[3 * id];
-            p[1] = a.x[3 * id + 1];
-            p[2] = a.x[3 * id + 2];
-        }
-        float g[3];
-#pragma unroll 1
-        for (int ax = 0; ax < 3; ++ax) {
-            float pp[3] = {p[0] + 0.0f, p[1] + 0.0f, p[2] + 0.0f}, pm[3] = {p[0] - 0.0f, p[1] - 0.0f, p[2] - 0.0f};
-            pp[ax] = p[ax] + kFdH;
-            pm[ax] = p[ax] - kFdH;
-            const float fp = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pp[0], pp[1], pp[2]);
-            const float fm = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, pm[0], pm[1], pm[2]);
-            g[ax] = (fp - fm) / kFdTwoH;
-        }
-        if (valid) {
-            nevals += 6;
-            normalize_grad(g[0], g[1], g[2], a.normal[3 * id], a.normal[3 * id + 1], a.normal[3 * id + 2]);
-        }
-    }
-    add_evals(a, 0, nevals, lane);
-}
 
 // ---------------------------------------------------------------------------
 // Pooled trace: the rays of several requests (each its own model and LOD) in
@@ -732,77 +692,6 @@ __global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a,
     add_evals(a, 0, nevals, lane);
 }
 
-// Normals over the pool: FP32 central differences of every hit.
-template <bool POSINV>
-__global__ __launch_bounds__(kPoolFp32Block) void pool_normals_kernel(TraceArgs a, PoolArgs p) {
-    __shared__ PoolFp32Slot slots[kPoolFp32Block / 64];
-    PoolFp32Slot* sl = &slots[threadIdx.x >> 6];
-    const uint32_t lane = lane_id();
-    int held = -1;
-    uint32_t nevals = 0;
-    bool drained = false;
-    for (;;) {
-        // one virtual id per idle lane from one counter, in order
-        int64_t slot = -1;
-        int rq = 0;
-        for (;;) {
-            const bool want = slot < 0 && !drained;
-            const uint64_t mask = __ballot(want);
-            if (mask == 0) break;
-            const int cnt = __popcll(mask);
-            const int leader = __ffsll((unsigned long long)mask) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(a.normal_queue, (uint32_t)cnt);
-            base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
-            if (want) {
-                const int64_t v = (int64_t)base + mask_rank(mask);
-                if (v < p.total) {
-                    const int r = pool_request_of(p, v);
-                    const int64_t sv = pool_slot_of(p, r, v);
-                    if (a.hit[sv]) {
-                        slot = sv;
-                        rq = r;
-                    }
-                }
-            }
-            if ((int64_t)base + cnt >= p.total) drained = true;
-        }
-        const bool valid = slot >= 0;
-        const uint64_t act = __ballot(valid);
-        if (act == 0) break;
-        float px = 0.0f, py = 0.0f, pz = 0.0f;
-        if (valid) {
-            px = a.x[3 * slot];
-            py = a.x[3 * slot + 1];
-            pz = a.x[3 * slot + 2];
-        }
-        float g3[3] = {0.0f, 0.0f, 0.0f};
-        uint64_t todo = act;
-        while (todo) {
-            const int g = __builtin_amdgcn_readlane(rq, __ffsll((unsigned long long)todo) - 1);
-            const bool in = valid && rq == g;
-            todo &= ~__ballot(in);
-            hold_request(p, g, held, sl, lane, stage_pool_fp32);
-            const int lod = p.req[g].lod;
-            const PoolLevels32 lv = pool_levels32(p, g);
-#pragma unroll 1
-            for (int ax = 0; ax < 3; ++ax) {
-                float pp[3] = {px + 0.0f, py + 0.0f, pz + 0.0f}, pm[3] = {px - 0.0f, py - 0.0f, pz - 0.0f};
-                const float c = ax == 0 ? px : (ax == 1 ? py : pz);
-                pp[ax] = c + kFdH;
-                pm[ax] = c - kFdH;
-                const float fp = sdf_fp32_wave_lv<POSINV>(lv, lod, &sl->dec, sl->b1, lane, in, pp[0], pp[1], pp[2]);
-                const float fm = sdf_fp32_wave_lv<POSINV>(lv, lod, &sl->dec, sl->b1, lane, in, pm[0], pm[1], pm[2]);
-                if (in) g3[ax] = (fp - fm) / kFdTwoH;
-            }
-        }
-        if (valid) {
-            nevals += 6;
-            normalize_grad(g3[0], g3[1], g3[2], a.normal[3 * slot], a.normal[3 * slot + 1], a.normal[3 * slot + 2]);
-        }
-    }
-    add_evals(a, 0, nevals, lane);
-}
 
 // Clears the hand-off flags of the launch's slots (one thread per virtual id)
 // and the claim counters.
@@ -842,14 +731,21 @@ __global__ void clear_round_kernel(uint32_t* flags, int words, uint32_t* counter
 constexpr int kMixedRounds = 3;
 
 template <bool POSINV>
+// The mixed form's normals: pass 1 hands every ray it finishes as a hit to
+// pass 2 as a PH_N0 record (defer_normals = 1), and pass 2 takes the six FP32
+// taps in-lane after the continuation (defer_normals = 0), so its drain
+// fills with tap work and there is no third launch.  The taps are the FP32
+// evaluations a separate normals pass would make (same bits).
 static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& timer) {
     const int g1 = grid_for(trace_mixed_pass1_kernel<POSINV, false>, n, kMixedBlock);
     const int g1r = grid_for(trace_mixed_pass1_kernel<POSINV, true>, n, kMixedBlock);
     const int g2 = grid_for(trace_mixed_pass2_kernel<POSINV, false>, n, kPass2Block);
-    const int g3 = grid_for(normals_pass_kernel<POSINV>, n, kPass2Block);
     const int words = (int)((n + 3) / 4);
     const dim3 cg((unsigned)((words + 255) / 256));
     const int rounds = a.return_dist > 0.0f ? kMixedRounds : 1;
+    TraceArgs a1 = a, a2 = a;  // pass 1 / pass 2 arguments
+    a1.defer_normals = 1;
+    a2.defer_normals = 0;
     for (int r = 0; r < rounds; ++r) {
         const bool allow_return = r + 1 < rounds;
         hipLaunchKernelGGL(clear_round_kernel, cg, dim3(256), 0, s, (uint32_t*)a.hflag, words, a.handoff_queue);
@@ -857,9 +753,9 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
         if (rc) return rc;
         timer.mark(kPassOther);
         if (r == 0)
-            hipLaunchKernelGGL((trace_mixed_pass1_kernel<POSINV, false>), dim3(g1), dim3(kMixedBlock), 0, s, a);
+            hipLaunchKernelGGL((trace_mixed_pass1_kernel<POSINV, false>), dim3(g1), dim3(kMixedBlock), 0, s, a1);
         else
-            hipLaunchKernelGGL((trace_mixed_pass1_kernel<POSINV, true>), dim3(g1r), dim3(kMixedBlock), 0, s, a);
+            hipLaunchKernelGGL((trace_mixed_pass1_kernel<POSINV, true>), dim3(g1r), dim3(kMixedBlock), 0, s, a1);
         rc = check_launch("trace_mixed_pass1_kernel");
         if (rc) return rc;
         timer.mark(kPassMarch);
@@ -870,18 +766,12 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
             timer.mark(kPassOther);
         }
         if (allow_return)
-            hipLaunchKernelGGL((trace_mixed_pass2_kernel<POSINV, true>), dim3(g2), dim3(kPass2Block), 0, s, a);
+            hipLaunchKernelGGL((trace_mixed_pass2_kernel<POSINV, true>), dim3(g2), dim3(kPass2Block), 0, s, a2);
         else
-            hipLaunchKernelGGL((trace_mixed_pass2_kernel<POSINV, false>), dim3(g2), dim3(kPass2Block), 0, s, a);
+            hipLaunchKernelGGL((trace_mixed_pass2_kernel<POSINV, false>), dim3(g2), dim3(kPass2Block), 0, s, a2);
         rc = check_launch("trace_mixed_pass2_kernel");
         if (rc) return rc;
         timer.mark(kPassFp32);
-    }
-    if (a.normals) {
-        hipLaunchKernelGGL(normals_pass_kernel<POSINV>, dim3(g3), dim3(kPass2Block), 0, s, a);
-        int rc = check_launch("normals_pass_kernel");
-        if (rc) return rc;
-        timer.mark(kPassNormals);
     }
     return NGLOD_OK;
 }
@@ -893,29 +783,24 @@ int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& 
 template <bool POSINV>
 static int trace_pool_passes(const TraceArgs& a, const PoolArgs& p, hipStream_t s, PassTimer& timer) {
     const int64_t n = p.total;
+    TraceArgs a1 = a, a2 = a;  // normals as in the dense trace: taken in pass 2
+    a1.defer_normals = 1;
+    a2.defer_normals = 0;
     hipLaunchKernelGGL(pool_clear_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, a.hflag, a.queue,
                        4 * kQueues);
     int rc = check_launch("pool_clear_kernel");
     if (rc) return rc;
     timer.mark(kPassOther);
     hipLaunchKernelGGL(pool_pass1_kernel<POSINV>, dim3(grid_for(pool_pass1_kernel<POSINV>, n, kMixedBlock)),
-                       dim3(kMixedBlock), 0, s, a, p);
+                       dim3(kMixedBlock), 0, s, a1, p);
     rc = check_launch("pool_pass1_kernel");
     if (rc) return rc;
     timer.mark(kPassMarch);
     hipLaunchKernelGGL(pool_pass2_kernel<POSINV>, dim3(grid_for(pool_pass2_kernel<POSINV>, n, kPoolFp32Block)),
-                       dim3(kPoolFp32Block), 0, s, a, p);
+                       dim3(kPoolFp32Block), 0, s, a2, p);
     rc = check_launch("pool_pass2_kernel");
     if (rc) return rc;
     timer.mark(kPassFp32);
-    if (a.normals) {
-        hipLaunchKernelGGL(pool_normals_kernel<POSINV>,
-                           dim3(grid_for(pool_normals_kernel<POSINV>, n, kPoolFp32Block)), dim3(kPoolFp32Block), 0, s,
-                           a, p);
-        rc = check_launch("pool_normals_kernel");
-        if (rc) return rc;
-        timer.mark(kPassNormals);
-    }
     return NGLOD_OK;
 }
 

@@ -126,13 +126,12 @@ int nglod_trace_pool(const nglod_model_t* models, const void* model_table, int32
     a.queue = (uint32_t*)counters;
     a.handoff_queue = a.queue + kQueues;
     a.ret_queue = a.queue + 2 * kQueues;
-    a.normal_queue = a.queue + 3 * kQueues;
     a.handoff = (Handoff*)workspace;
     a.hflag = (uint8_t*)workspace + (size_t)pool_rays * sizeof(Handoff);
     a.ret = nullptr;
     a.rflag = nullptr;
     a.return_dist = 0.0f;
-    a.defer_normals = 1;
+    a.defer_normals = 0;  // the pooled passes set their own
     a.solo_lanes = solo_lanes_setting();
     a.coarse = CoarsePlan{0, 0};
     a.replay_steps = params->refine_steps;

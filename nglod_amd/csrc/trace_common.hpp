@@ -34,7 +34,7 @@ __device__ __forceinline__ void normalize_grad(float g0, float g1, float g2, flo
 // (workgroups are dispatched to XCDs round-robin, so blockIdx % 8 picks the
 // XCD-local range and its L2 sees one band of the image); a wave that drains
 // its own range moves on to the next ones.
-// Threads per block of the FP32 march and the normals pass: one block of 8
+// Threads per block of the FP32 march: one block of 8
 // waves per CU (the 128 KB of decoder and coarse-level tables in LDS leave room
 // for one), sharing those tables.  Measured on the 1280x720 LOD-4 frame: 512
 // threads 4.66 ms, 768 4.99, 384 5.10, 1024 5.36, 256 5.68 (more waves per CU
@@ -86,9 +86,9 @@ struct TraceArgs {
     struct Handoff* ret;      // rays handed back from the FP32 pass to the mixed march
     uint8_t* rflag;
     uint32_t* ret_queue;      // [kQueues] claim counters of a pass 1 that resumes them
-    uint32_t* normal_queue;   // [kQueues] claim counters of the normals pass
     float return_dist;        // FP32 step length above which a ray goes back to mixed
-    int defer_normals;        // 1: a finished hit is written out without its normal (a later pass computes it)
+    int defer_normals;        // 0: a finished hit takes its normal taps in-lane; 1: it is handed to the FP32
+                              // pass (hand-off record in phase PH_N0), which takes the taps
     int solo_lanes;           // FP32 passes: a wave with no rays left to claim and at most this many busy
                               // lanes evaluates them one at a time with the whole wave (sdf_fp32_solo)
     int64_t nrays;  // the frame is split into kQueues ranges [nrays*q/kQueues, nrays*(q+1)/kQueues)
@@ -132,19 +132,14 @@ struct RayState {
     }
 
     // After the march: hit &= all(|x| <= 1) (SphereTracer.py:119), then
-    // either start the six normal taps or write the ray out.
+    // start the six normal taps (in-lane, or in the FP32 pass the hit is
+    // handed to) or write the ray out.
     __device__ __forceinline__ void finish(const TraceArgs& a) {
         const bool outside = (fabsf(x0) > 1.0f) || (fabsf(x1) > 1.0f) || (fabsf(x2) > 1.0f);
         const bool hit = hitflag && !outside;
         if (hit && a.normals && a.defer_normals) {
-            a.x[3 * ray] = x0;
-            a.x[3 * ray + 1] = x1;
-            a.x[3 * ray + 2] = x2;
-            a.t[ray] = t;
-            a.hit[ray] = 1;
-            if (a.ray_steps) a.ray_steps[ray] = steps;
-            phase = PH_IDLE;
-            ray = -1;
+            phase = PH_N0;
+            hand_off_to(a.handoff, a.hflag);
         } else if (hit && a.normals) {
             phase = PH_N0;
             qx = x0 + kFdH;

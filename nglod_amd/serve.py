@@ -11,8 +11,8 @@ memory) and report the same statistics:
   takes a range of ray slots.  Every admitted request that is waiting when a
   launch slot frees up joins the next batch, and a batch is one
   nglod_trace_pool call: one persistent mixed-precision march over the rays
-  of all its requests (each with its own model and LOD), one FP32
-  continuation pass and one normals pass.  Up to `slots` batches are in
+  of all its requests (each with its own model and LOD) and one FP32
+  continuation pass, which also takes the normals.  Up to `slots` batches are in
   flight, each on its own stream, so requests that arrive while a batch runs
   start in a new batch at once when a batch slot is free, and the passes of
   two batches overlap on the device (one batch's drain tail under the

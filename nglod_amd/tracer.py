@@ -182,7 +182,8 @@ class SphereTracer(BaseTracer):
         self._ws = {}
         self.last_evals = None
         # When True, each call times its passes with HIP events (and waits
-        # for them): last_pass_ms = [march, FP32 continuation, normals, resets].
+        # for them): last_pass_ms = [march, FP32 continuation (with the
+        # mixed form's normals), 0 (no separate normals pass), resets].
         self.time_passes = False
         self.last_pass_ms = None
 
