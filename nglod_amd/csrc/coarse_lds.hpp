@@ -102,16 +102,34 @@ __device__ __forceinline__ void gather_fp32_coarse(const ModelArgs& m, int lod, 
             corner_fma_lds(f, s, i0 + sz + sy, wxy01 * az.w1);
             corner_fma_lds(f, s, i0 + sz + sy + 1, wxy11 * az.w1);
         } else {
+            // global levels: the corner loads in two batches of four corners
+            // ahead of their FMAs (as gather_fp32_batched; same order)
             const float* base = m.grid[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
             const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
-            corner_fma(f, base, wxy00 * az.w0);
-            corner_fma(f, base + sx, wxy10 * az.w0);
-            corner_fma(f, base + sy, wxy01 * az.w0);
-            corner_fma(f, base + sy + sx, wxy11 * az.w0);
-            corner_fma(f, base + sz, wxy00 * az.w1);
-            corner_fma(f, base + sz + sx, wxy10 * az.w1);
-            corner_fma(f, base + sz + sy, wxy01 * az.w1);
-            corner_fma(f, base + sz + sy + sx, wxy11 * az.w1);
+            const float* cp[8] = {base, base + sx, base + sy, base + sy + sx,
+                                  base + sz, base + sz + sx, base + sz + sy, base + sz + sy + sx};
+            const float w[8] = {wxy00 * az.w0, wxy10 * az.w0, wxy01 * az.w0, wxy11 * az.w0,
+                                wxy00 * az.w1, wxy10 * az.w1, wxy01 * az.w1, wxy11 * az.w1};
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                float4 g[4][kC / 4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int v = 0; v < kC / 4; ++v) g[c][v] = reinterpret_cast<const float4*>(cp[4 * b + c])[v];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                    for (int v = 0; v < kC / 4; ++v) {
+                        f[4 * v + 0] = __builtin_fmaf(g[c][v].x, w[4 * b + c], f[4 * v + 0]);
+                        f[4 * v + 1] = __builtin_fmaf(g[c][v].y, w[4 * b + c], f[4 * v + 1]);
+                        f[4 * v + 2] = __builtin_fmaf(g[c][v].z, w[4 * b + c], f[4 * v + 2]);
+                        f[4 * v + 3] = __builtin_fmaf(g[c][v].w, w[4 * b + c], f[4 * v + 3]);
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
 }

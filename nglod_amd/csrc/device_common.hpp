@@ -158,6 +158,54 @@ __device__ __forceinline__ void gather_fp32(const LV& m, int lod, float px, floa
     }
 }
 
+// gather_fp32 with the corner loads issued in two batches of four corners
+// (32 16-byte loads each) ahead of their FMAs, so that a wave waits about two
+// memory latencies per level instead of one per handful of loads (a wave-wide
+// load completes with its slowest lane, and on an L1 miss that is an L2 round
+// trip).  Same corners, weights and FMA order as gather_fp32: the same bits.
+// For kernels with registers to spare (128 VGPRs hold a batch).
+template <typename LV>
+__device__ __forceinline__ void gather_fp32_batched(const LV& m, int lod, float px, float py, float pz,
+                                                    float (&f)[kC]) {
+#pragma unroll
+    for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+    for (int l = 0; l <= lod; ++l) {
+        const int r = m.res[l];
+        const int S = r + 1;
+        const AxisW ax = axis_weights(px, r);
+        const AxisW ay = axis_weights(py, r);
+        const AxisW az = axis_weights(pz, r);
+        const float* base = m.grid[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
+        const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
+        const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
+        const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
+        const float* cp[8] = {base, base + sx, base + sy, base + sy + sx,
+                              base + sz, base + sz + sx, base + sz + sy, base + sz + sy + sx};
+        const float w[8] = {wxy00 * az.w0, wxy10 * az.w0, wxy01 * az.w0, wxy11 * az.w0,
+                            wxy00 * az.w1, wxy10 * az.w1, wxy01 * az.w1, wxy11 * az.w1};
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            float4 g[4][kC / 4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+#pragma unroll
+                for (int v = 0; v < kC / 4; ++v) g[c][v] = reinterpret_cast<const float4*>(cp[4 * b + c])[v];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+#pragma unroll
+                for (int v = 0; v < kC / 4; ++v) {
+                    f[4 * v + 0] = __builtin_fmaf(g[c][v].x, w[4 * b + c], f[4 * v + 0]);
+                    f[4 * v + 1] = __builtin_fmaf(g[c][v].y, w[4 * b + c], f[4 * v + 1]);
+                    f[4 * v + 2] = __builtin_fmaf(g[c][v].z, w[4 * b + c], f[4 * v + 2]);
+                    f[4 * v + 3] = __builtin_fmaf(g[c][v].w, w[4 * b + c], f[4 * v + 3]);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
 // relu(e W0^T + b0) W1^T + b1 with e = [xyz, f] (POSINV=false) or f
 // (OctreeSDF.py:79-88, :113-115).  Weights are wave-uniform: scalar loads.
 template <bool POSINV>

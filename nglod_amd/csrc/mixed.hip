@@ -244,7 +244,14 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
             break;
         }
         const bool busy = s.phase != PH_IDLE;
-        const float d = sdf_mixed_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        __half2 acc[kC / 2];
+        if (busy) {
+            gather_fp16_batched(a.m, a.lod, s.qx, s.qy, s.qz, acc);
+        } else {
+#pragma unroll
+            for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
+        }
+        const float d = decode_mixed_wave<POSINV>(acc, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
             hist.push(s);
@@ -308,7 +315,14 @@ __global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void trace_mixed_pass
             break;
         }
         const bool busy = s.phase != PH_IDLE;
-        const float d = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        float f[kC];
+        if (busy) {
+            gather_fp32_batched(a.m, a.lod, s.qx, s.qy, s.qz, f);
+        } else {
+#pragma unroll
+            for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+        }
+        const float d = decode_fp32_wave<POSINV>(f, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
             s.consume(a, d);
