@@ -221,9 +221,10 @@ typedef struct nglod_pool_request {
  * disjoint slots may share it, also while they overlap. */
 size_t nglod_trace_pool_workspace_size(int64_t pool_rays);
 
-/* Device bytes of one nglod_trace_pool call's claim counters; calls that may
- * overlap (on different streams) need their own. */
-#define NGLOD_POOL_COUNTER_BYTES 256
+/* Device bytes of one nglod_trace_pool call's claim counters (one 128-byte
+ * line per counter: counters sharing a line slowed every claim); calls that
+ * may overlap (on different streams) need their own. */
+#define NGLOD_POOL_COUNTER_BYTES 4096
 
 /* nglod_trace_dense (NGLOD_MIXED, SphereTracer semantics) for up to 64
  * requests at once, each with its own model (an index into model_table;
@@ -506,7 +507,7 @@ int nglod_sol_trinkets(const int16_t* points, const int32_t* pyramid, int32_t le
 int nglod_sol_sdf(const nglod_sol_t* sol, const float* x, const int32_t* pidx, int64_t n, float* out,
                   void* stream);
 
-/* Workspace bytes of nglod_trace_sparse (a queue counter). */
+/* Workspace bytes of nglod_trace_sparse (its claim counters). */
 size_t nglod_trace_sparse_workspace_size(int64_t n);
 
 /* Sparse sphere march with normals (SDF::sphereTrace + SDF::getNormal,

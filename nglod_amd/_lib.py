@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnglod_hip.so")
 
 MAX_LODS = 8
+POOL_COUNTER_BYTES = 4096  # NGLOD_POOL_COUNTER_BYTES
 
 OK = 0
 ERR_ARG = -1

@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kMarchBlock) void trace_dense_fp32_kernel(TraceArgs
     const uint32_t lane = lane_id();
     float* sv = solo_sv[threadIdx.x >> 6];
     RayFeed feed;
-    feed.init(a.nrays);
+    feed.init(a.nrays, a.queue_stride);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
@@ -329,9 +329,10 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
 }
 
 // Workspace: kWsCounters bytes of counters (the ray queues, then the claim
-// counters of pass 2 and of a pass 1 over returned rays, kQueues each); for the mixed form then hand-off records [n], returned records [n], hand-off flags
+// counters of pass 2 and of a pass 1 over returned rays, kQueues each, one
+// 128-byte line per counter); for the mixed form then hand-off records [n], returned records [n], hand-off flags
 // [n] and return flags [n] (each flag array padded to 256 bytes).
-static constexpr int kCounterWords = 3 * kQueues;
+static constexpr int kCounterWords = 3 * kQueues * kQueueStride;
 static constexpr size_t kWsCounters = (4 * kCounterWords + 255) / 256 * 256;
 
 static size_t flag_bytes(int64_t n) { return ((size_t)n + 255) / 256 * 256; }
@@ -394,8 +395,9 @@ static int trace_dense_run(const nglod_model_t* model, const ModelArgs& m, const
     a.hit = hit;
     a.normal = normal;
     a.queue = w.counters;
-    a.handoff_queue = a.queue + kQueues;
-    a.ret_queue = a.queue + 2 * kQueues;
+    a.queue_stride = kQueueStride;
+    a.handoff_queue = a.queue + kQueues * kQueueStride;
+    a.ret_queue = a.queue + 2 * kQueues * kQueueStride;
     a.handoff = w.handoff;
     a.ret = w.ret;
     a.hflag = w.hflag;

@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
     const uint32_t lane = lane_id();
     const float b1 = a.m.b1[a.dec][0];
     RayFeed feed;
-    feed.init(a.nrays);
+    feed.init(a.nrays, a.queue_stride);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void trace_mixed_pass
     const uint32_t lane = lane_id();
     float* sv = solo_sv[threadIdx.x >> 6];
     RayFeed feed;
-    feed.init(a.nrays);
+    feed.init(a.nrays, a.queue_stride);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
@@ -512,7 +512,7 @@ __global__ __launch_bounds__(kMixedBlock, 3) void pool_pass1_kernel(TraceArgs a,
     const uint32_t lane = lane_id();
     int held = -1;  // request whose decoder this wave's slot holds (wave-uniform)
     RayFeed feed;
-    feed.init(p.total);
+    feed.init(p.total, a.queue_stride);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a,
     const uint32_t lane = lane_id();
     int held = -1;
     RayFeed feed;
-    feed.init(p.total);
+    feed.init(p.total, a.queue_stride);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
@@ -708,10 +708,10 @@ __global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a,
 
 
 // Clears the hand-off flags of the launch's slots (one thread per virtual id)
-// and the claim counters.
-__global__ void pool_clear_kernel(PoolArgs p, uint8_t* hflag, uint32_t* counters, int ncounters) {
+// and the ncounters claim counters (`stride` words apart).
+__global__ void pool_clear_kernel(PoolArgs p, uint8_t* hflag, uint32_t* counters, int ncounters, int stride) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < ncounters) counters[v] = 0u;
+    if (v < ncounters) counters[v * stride] = 0u;
     if (v < p.total) {
         const int r = pool_request_of(p, v);
         hflag[pool_slot_of(p, r, v)] = 0;
@@ -733,11 +733,12 @@ __global__ void clear_words_kernel(uint32_t* p, int n) {
     if (i < n) p[i] = 0u;
 }
 
-// Clears `words` uint32 of flags and the kQueues claim counters of a pass.
-__global__ void clear_round_kernel(uint32_t* flags, int words, uint32_t* counters) {
+// Clears `words` uint32 of flags and the kQueues claim counters of a pass
+// (`stride` words apart).
+__global__ void clear_round_kernel(uint32_t* flags, int words, uint32_t* counters, int stride) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < words) flags[i] = 0u;
-    if (i < kQueues) counters[i] = 0u;
+    if (i < kQueues) counters[i * stride] = 0u;
 }
 
 // Rounds of (pass 1, pass 2) when near misses may go back to mixed precision
@@ -762,7 +763,7 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
     a2.defer_normals = 0;
     for (int r = 0; r < rounds; ++r) {
         const bool allow_return = r + 1 < rounds;
-        hipLaunchKernelGGL(clear_round_kernel, cg, dim3(256), 0, s, (uint32_t*)a.hflag, words, a.handoff_queue);
+        hipLaunchKernelGGL(clear_round_kernel, cg, dim3(256), 0, s, (uint32_t*)a.hflag, words, a.handoff_queue, a.queue_stride);
         int rc = check_launch("clear_round_kernel");
         if (rc) return rc;
         timer.mark(kPassOther);
@@ -774,7 +775,7 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
         if (rc) return rc;
         timer.mark(kPassMarch);
         if (allow_return) {
-            hipLaunchKernelGGL(clear_round_kernel, cg, dim3(256), 0, s, (uint32_t*)a.rflag, words, a.ret_queue);
+            hipLaunchKernelGGL(clear_round_kernel, cg, dim3(256), 0, s, (uint32_t*)a.rflag, words, a.ret_queue, a.queue_stride);
             rc = check_launch("clear_round_kernel");
             if (rc) return rc;
             timer.mark(kPassOther);
@@ -801,7 +802,7 @@ static int trace_pool_passes(const TraceArgs& a, const PoolArgs& p, hipStream_t 
     a1.defer_normals = 1;
     a2.defer_normals = 0;
     hipLaunchKernelGGL(pool_clear_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, a.hflag, a.queue,
-                       4 * kQueues);
+                       3 * kQueues, a.queue_stride);
     int rc = check_launch("pool_clear_kernel");
     if (rc) return rc;
     timer.mark(kPassOther);

@@ -123,9 +123,12 @@ int nglod_trace_pool(const nglod_model_t* models, const void* model_table, int32
     a.t = t;
     a.hit = hit;
     a.normal = normal;
+    static_assert(3 * kQueues * kQueueStride * sizeof(uint32_t) <= NGLOD_POOL_COUNTER_BYTES,
+                  "the pool's claim counters fit NGLOD_POOL_COUNTER_BYTES");
     a.queue = (uint32_t*)counters;
-    a.handoff_queue = a.queue + kQueues;
-    a.ret_queue = a.queue + 2 * kQueues;
+    a.queue_stride = kQueueStride;
+    a.handoff_queue = a.queue + kQueues * kQueueStride;
+    a.ret_queue = a.queue + 2 * kQueues * kQueueStride;
     a.handoff = (Handoff*)workspace;
     a.hflag = (uint8_t*)workspace + (size_t)pool_rays * sizeof(Handoff);
     a.ret = nullptr;

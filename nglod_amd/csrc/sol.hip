@@ -329,7 +329,7 @@ __global__ __launch_bounds__(kSparseBlock) void trace_sparse_kernel(SolArgs a) {
     const float b1 = a.b1[0];
     const uint32_t lane = lane_id();
     RayFeed feed;
-    feed.init(a.n);
+    feed.init(a.n, kQueueStride);
     SolRay s;
     s.ray = -1;
     s.phase = SP_IDLE;
@@ -390,10 +390,13 @@ __global__ void point_codes_kernel(const float* __restrict__ x, const uint8_t* _
     codes[i] = keep ? (uint32_t)morton_u64(c[0], c[1], c[2]) : 0xFFFFFFFFu;
 }
 
-// Zeroes the first kQueues counters of a workspace.
-__global__ void clear_counters_kernel(uint32_t* q) {
-    if (threadIdx.x < kQueues) q[threadIdx.x] = 0;
+// Zeroes kQueues counters of a workspace, `stride` words apart.
+__global__ void clear_counters_kernel(uint32_t* q, int stride) {
+    if (threadIdx.x < kQueues) q[threadIdx.x * stride] = 0;
 }
+
+// The sparse march's workspace: its claim counters, one 128-byte line each.
+constexpr size_t kSparseWorkspace = (size_t)kQueues * kQueueStride * sizeof(uint32_t);
 
 // ---------------------------------------------------------------------------
 // sparse structure builders
@@ -578,7 +581,7 @@ int nglod_sol_trinkets(const int16_t* points, const int32_t* pyramid, int32_t le
     const int32_t* pcnt = pyramid;
     const int32_t* psum = pyramid + (level + 2);
     uint32_t* missing = (uint32_t*)workspace;
-    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, s, missing);
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, s, missing, 1);
     int rc = check_launch("clear_counters_kernel");
     if (rc) return rc;
     int64_t off = 0;
@@ -641,7 +644,7 @@ int nglod_sol_sdf(const nglod_sol_t* sol, const float* x, const int32_t* pidx, i
 
 size_t nglod_trace_sparse_workspace_size(int64_t n) {
     (void)n;
-    return 256;
+    return kSparseWorkspace;
 }
 
 int nglod_trace_sparse(const nglod_sol_t* sol, const float* ray_o, const float* ray_d, int64_t n,
@@ -654,7 +657,7 @@ int nglod_trace_sparse(const nglod_sol_t* sol, const float* ray_o, const float* 
     if (n < 0 || n > (int64_t)0x7fff0000) return fail(NGLOD_ERR_ARG, "n out of range");
     if (n == 0) return NGLOD_OK;
     if (!ray_o || !ray_d || !ray_offsets || !x || !t || !hit || !normal) return fail(NGLOD_ERR_ARG, "NULL pointer");
-    if (!workspace || workspace_bytes < 256) return fail(NGLOD_ERR_WORKSPACE, "workspace too small");
+    if (!workspace || workspace_bytes < kSparseWorkspace) return fail(NGLOD_ERR_WORKSPACE, "workspace too small");
     hipStream_t s = (hipStream_t)stream;
     a.o = ray_o;
     a.d = ray_d;
@@ -668,7 +671,7 @@ int nglod_trace_sparse(const nglod_sol_t* sol, const float* ray_o, const float* 
     a.iters = iters;
     a.queue = (uint32_t*)workspace;
     a.evals = (unsigned long long*)evals_out;
-    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, s, a.queue);
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, s, a.queue, kQueueStride);
     rc = check_launch("clear_counters_kernel");
     if (rc) return rc;
     const bool posinv = a.din == kC;

@@ -42,6 +42,10 @@ __device__ __forceinline__ void normalize_grad(float g0, float g1, float g2, flo
 constexpr int kMarchBlock = 512;
 constexpr int kQueues = 8;  // a multiple of 8 (one share per XCD)
 static_assert(kQueues % 8 == 0 && kQueues <= 256, "kQueues: a multiple of 8, at most 256");
+// Stride (in words) of the single-request traces' claim counters: one
+// 128-byte line per counter.  Packed into one line, the counters of all eight
+// XCDs' ranges made every wave's claim atomic contend for that line.
+constexpr int kQueueStride = 32;
 
 // PH_REPLAY: the mixed-precision march stopped on a surface test; the kernel
 // rewinds the last steps and hands them to the FP32 pass.
@@ -82,10 +86,11 @@ struct TraceArgs {
     float near_dist;   // ... back to the last step of at least this length (0: all replay_steps)
     struct Handoff* handoff;  // rays handed from the mixed march to the FP32 pass
     uint8_t* hflag;
-    uint32_t* handoff_queue;  // [kQueues] claim counters of the FP32 pass
+    uint32_t* handoff_queue;  // [kQueues * queue_stride] claim counters of the FP32 pass
     struct Handoff* ret;      // rays handed back from the FP32 pass to the mixed march
     uint8_t* rflag;
-    uint32_t* ret_queue;      // [kQueues] claim counters of a pass 1 that resumes them
+    uint32_t* ret_queue;      // [kQueues * queue_stride] claim counters of a pass 1 that resumes them
+    int queue_stride;         // words between the claim counters of the kQueues ranges
     float return_dist;        // FP32 step length above which a ray goes back to mixed
     int defer_normals;        // 0: a finished hit takes its normal taps in-lane; 1: it is handed to the FP32
                               // pass (hand-off record in phase PH_N0), which takes the taps
@@ -370,12 +375,14 @@ __device__ __forceinline__ float fp32_solo_step(const TraceArgs& a, const Fp32Ld
 struct RayFeed {
     int64_t nrays;
     int q0, q, tried;
+    int stride;      // words between the ranges' claim counters
     bool drained;    // no ids left to claim (wave-uniform)
     int64_t cb, ce;  // the wave's unused ids of its current chunk (wave-uniform)
     uint64_t fmask;  // next_flagged: flagged ids cb + bit not yet handed out (wave-uniform)
 
-    __device__ __forceinline__ void init(int64_t n) {
+    __device__ __forceinline__ void init(int64_t n, int counter_stride = 1) {
         nrays = n;
+        stride = counter_stride;
         q0 = q = blockIdx.x % kQueues;
         tried = 0;
         drained = false;
@@ -401,7 +408,7 @@ struct RayFeed {
     __device__ __forceinline__ bool claim(uint32_t* counters, uint32_t lane, int leader, int take) {
         const int qi = SPLIT ? q : 0;
         uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&counters[qi], (uint32_t)take);
+        if ((int)lane == leader) base = atomicAdd(&counters[qi * stride], (uint32_t)take);
         base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
         const int64_t lo = SPLIT ? range_begin(qi) : 0;
         const int64_t qsize = (SPLIT ? range_begin(qi + 1) : nrays) - lo;
@@ -466,7 +473,7 @@ struct RayFeed {
                 const int leader = __ffsll((unsigned long long)mask) - 1;
                 uint32_t base = 0;
                 const int qi = SPLIT ? q : 0;
-                if ((int)lane == leader) base = atomicAdd(&counters[qi], (uint32_t)take);
+                if ((int)lane == leader) base = atomicAdd(&counters[qi * stride], (uint32_t)take);
                 base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader);
                 const int64_t lo = SPLIT ? range_begin(qi) : 0;
                 const int64_t qsize = (SPLIT ? range_begin(qi + 1) : nrays) - lo;

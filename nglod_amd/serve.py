@@ -208,7 +208,7 @@ class TraceServer:
         self.pws = torch.empty(int(lib.nglod_trace_pool_workspace_size(cap)), dtype=torch.uint8, device=dev)
         # one stream and one set of claim counters per batch in flight
         self.bstreams = [torch.cuda.Stream(device=dev) for _ in range(self.slots)]
-        self.counters = torch.empty(self.slots, 256, dtype=torch.uint8, device=dev)
+        self.counters = torch.empty(self.slots, _lib.POOL_COUNTER_BYTES, dtype=torch.uint8, device=dev)
         n = len(self.descs)
         self.desc_array = (_lib.NglodModel * n)(*self.descs)
         self.table = torch.empty(int(lib.nglod_model_table_bytes(n)), dtype=torch.uint8, device=dev)

@@ -108,7 +108,7 @@ class SolNGLOD:
                                           self.num_lods + 1, _lib.ptr(self.cc), ctypes.cast(cnt, ctypes.c_void_p),
                                           _lib.ptr(self.trinket_v), _lib.ptr(self.parent), _lib.ptr(ws), ws.numel(),
                                           _lib.stream_of(ws)), "nglod_sol_trinkets")
-        self._ws = torch.empty(256, dtype=torch.uint8, device=self.device)
+        self._ws = torch.empty(_lib.load().nglod_trace_sparse_workspace_size(0), dtype=torch.uint8, device=self.device)
         self.last_evals = None
 
     def _check_corner_blocks(self):

@@ -37,6 +37,13 @@ def test_python_binding_covers_every_symbol():
     assert set(declared_symbols()) <= set(_lib.SIGNATURES)
 
 
+def test_header_constants_match_the_binding():
+    text = open(os.path.join(ROOT, "include", "nglod_hip.h")).read()
+    defines = dict(re.findall(r"^#define\s+(NGLOD_\w+)\s+(-?\d+)\s*$", text, flags=re.M))
+    assert int(defines["NGLOD_MAX_LODS"]) == _lib.MAX_LODS
+    assert int(defines["NGLOD_POOL_COUNTER_BYTES"]) == _lib.POOL_COUNTER_BYTES
+
+
 def test_abi_version():
     assert _lib.load().nglod_abi_version() == 1
 
