@@ -310,7 +310,8 @@ int nglod_normals_fd(const nglod_model_t* model, const float* x, int64_t n, int3
                      void* stream) {
     // Normals are FP32 in both forms (finite differences of h = 1/192 need
     // more than BF16 resolution); the mixed form takes them from the FP32
-    // cumulative grid, as its trace's FP32 pass does.
+    // cumulative grid, as its trace's FP32 pass does (that pass decodes with
+    // exact BF16 splits on the matrix cores: FP32-level, not the same bits).
     ModelArgs m;
     int rc = make_model_args(model, lod, form, &m);
     if (rc) return rc;

@@ -23,6 +23,7 @@
 #include "api_common.hpp"
 #include "fp32_wave.hpp"
 #include "mixed_device.hpp"
+#include "split_wave.hpp"
 #include "trace_common.hpp"
 
 namespace nglod {
@@ -289,40 +290,32 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
 // round); a separate instantiation, as for pass 1.
 template <bool POSINV, bool RETURNS>
 __global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void trace_mixed_pass2_kernel(TraceArgs a) {
-    __shared__ Fp32Lds lds;
-    __shared__ __align__(16) float solo_sv[kPass2Block / 64][kH];
+    __shared__ SplitLds lds;
     __shared__ SoloLevels solo_lv;
-    stage_fp32_lds(a.m, a.dec, &lds);
+    stage_split_lds(a.m, a.dec, &lds);
     stage_solo_levels(a.m, &solo_lv);
     __syncthreads();
     const float b1 = a.m.b1[a.dec][0];
     const uint32_t lane = lane_id();
-    float* sv = solo_sv[threadIdx.x >> 6];
     RayFeed feed;
     feed.init(a.nrays, a.queue_stride);
     RayState s;
     s.ray = -1;
     s.phase = PH_IDLE;
     uint32_t nevals = 0;
+    const int coop = a.solo_lanes < kCoopLanes ? a.solo_lanes : kCoopLanes;
     bool tail = false;
     for (;;) {
         const int64_t id = feed.next_flagged<true>(a.handoff_queue, a.hflag, lane, s.phase == PH_IDLE);
         if (id >= 0) s.resume(a, a.handoff[id]);
         const uint64_t act = __ballot(s.phase != PH_IDLE);
         if (act == 0) break;
-        if (feed.drained && feed.fmask == 0 && __popcll(act) <= a.solo_lanes) {
+        if (feed.drained && feed.fmask == 0 && __popcll(act) <= coop) {
             tail = true;
             break;
         }
         const bool busy = s.phase != PH_IDLE;
-        float f[kC];
-        if (busy) {
-            gather_fp32_batched(a.m, a.lod, s.qx, s.qy, s.qz, f);
-        } else {
-#pragma unroll
-            for (int c = 0; c < kC; ++c) f[c] = 0.0f;
-        }
-        const float d = decode_fp32_wave<POSINV>(f, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        const float d = sdf_split_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
             s.consume(a, d);
@@ -331,13 +324,17 @@ __global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void trace_mixed_pass
                 s.hand_off_to(a.ret, a.rflag);
         }
     }
-    // the wave's last few rays, one query at a time (fp32_solo_step)
+    // The wave's last few rays: the same steps with the gather spread over the
+    // idle lanes (same bits).  A separate loop, so that its registers do not
+    // weigh on the main loop's.
     while (tail) {
         const uint64_t act = __ballot(s.phase != PH_IDLE);
         if (act == 0) break;
-        const float d = fp32_solo_step<POSINV>(a, &lds, nullptr, CoarsePlan{0, 0}, &solo_lv, sv, b1, lane, act,
-                                               s.qx, s.qy, s.qz);
-        if (s.phase != PH_IDLE) {
+        const bool busy = s.phase != PH_IDLE;
+        float f[kC];
+        gather_fp32_coop(&solo_lv, a.lod, lane, act, s.qx, s.qy, s.qz, f);
+        const float d = decode_split_wave<POSINV>(f, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        if (busy) {
             ++nevals;
             s.consume(a, d);
             if (RETURNS && s.phase == PH_MARCH && fabsf(s.D) > a.return_dist && fabsf(s.D) > fabsf(s.Dp))
@@ -428,7 +425,6 @@ struct PoolFp32Slot {
     Fp32Lds dec;
     SoloLevels lv;
     float b1;
-    __align__(16) float sv[kH];
 };
 
 // Fills a wave's mixed slot for request e (the wave's 64 lanes).  Out of
@@ -651,6 +647,7 @@ __global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a,
     s.phase = PH_IDLE;
     int rq = 0;
     uint32_t nevals = 0;
+    const int coop = a.solo_lanes < kCoopLanes ? a.solo_lanes : kCoopLanes;
     bool tail = false;
     for (;;) {
         int r = 0;
@@ -661,7 +658,7 @@ __global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a,
         }
         const uint64_t act = __ballot(s.phase != PH_IDLE);
         if (act == 0) break;
-        if (feed.drained && feed.fmask == 0 && __popcll(act) <= a.solo_lanes) {
+        if (feed.drained && feed.fmask == 0 && __popcll(act) <= coop) {
             tail = true;
             break;
         }
@@ -673,8 +670,8 @@ __global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a,
             const bool in = busy && rq == g;
             todo &= ~__ballot(in);
             hold_request(p, g, held, sl, lane, stage_pool_fp32);
-            const float dg = sdf_fp32_wave_lv<POSINV>(pool_levels32(p, g), p.req[g].lod, &sl->dec, sl->b1, lane, in,
-                                                      s.qx, s.qy, s.qz);
+            const float dg = sdf_split_wave<POSINV>(pool_levels32(p, g), p.req[g].lod, &sl->dec, sl->b1, lane, in,
+                                                    s.qx, s.qy, s.qz);
             if (in) d = dg;
         }
         if (busy) {
@@ -682,21 +679,24 @@ __global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a,
             s.consume(a, d);
         }
     }
-    // the wave's last few rays, one query at a time with the whole wave
+    // the wave's last few rays, request by request: the gather spread over
+    // the idle lanes (gather_fp32_coop), the same decoder (same bits)
     while (tail) {
         const uint64_t act = __ballot(s.phase != PH_IDLE);
         if (act == 0) break;
+        const bool busy = s.phase != PH_IDLE;
         float d = 0.0f;
-        uint64_t mm = act;
-        while (mm) {
-            const int q = __ffsll((unsigned long long)mm) - 1;
-            mm &= mm - 1ull;
-            const int g = __builtin_amdgcn_readlane(rq, q);
+        uint64_t todo = act;
+        while (todo) {
+            const int g = __builtin_amdgcn_readlane(rq, __ffsll((unsigned long long)todo) - 1);
+            const bool in = busy && rq == g;
+            const uint64_t gm = __ballot(in);
+            todo &= ~gm;
             hold_request(p, g, held, sl, lane, stage_pool_fp32);
-            const float px = __shfl(s.qx, q), py = __shfl(s.qy, q), pz = __shfl(s.qz, q);
-            const float dq = sdf_fp32_solo<POSINV>(&sl->lv, p.req[g].lod, &sl->dec, sl->b1, lane, sl->sv, px, py, pz,
-                                                   CoarsePlan{0, 0}, nullptr);
-            if ((int)lane == q) d = dq;
+            float f[kC];
+            gather_fp32_coop(&sl->lv, p.req[g].lod, lane, gm, s.qx, s.qy, s.qz, f);
+            const float dg = decode_split_wave<POSINV>(f, &sl->dec, sl->b1, lane, in, s.qx, s.qy, s.qz);
+            if (in) d = dg;
         }
         if (s.phase != PH_IDLE) {
             ++nevals;

@@ -151,8 +151,10 @@ __device__ __forceinline__ uint32_t pack_bf16(__half2 h) {
 
 // b0 of this lane's 16 rows of hidden tile h (rows 32h + 8g + 4 half + 0..3,
 // g = 0..3): the first MFMA of the tile accumulates onto it, so the bias
-// costs no vector instruction.
-__device__ __forceinline__ f32x16 bias_tile(int h, uint32_t half, const MixedLds* s) {
+// costs no vector instruction.  LDS: any staged decoder with b0[], w1[] and
+// wxyz[][4] (MixedLds, SplitLds).
+template <typename LDS>
+__device__ __forceinline__ f32x16 bias_tile(int h, uint32_t half, const LDS* s) {
     f32x16 c;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
@@ -177,7 +179,8 @@ __device__ __forceinline__ float relu_bits(float v) {
 // Second layer over one 32x32 output tile held in registers (the first
 // layer's output with b0 already in it): sum of
 // w1[row] * relu(C + wxyz[row] . xyz) over this lane's 16 rows.
-__device__ __forceinline__ float layer2_partial(const f32x16& c, int h, uint32_t half, const MixedLds* s, bool xyz,
+template <typename LDS>
+__device__ __forceinline__ float layer2_partial(const f32x16& c, int h, uint32_t half, const LDS* s, bool xyz,
                                                 float px, float py, float pz) {
     float acc = 0.0f;
 #pragma unroll

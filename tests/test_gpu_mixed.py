@@ -117,6 +117,28 @@ def test_trace_mixed_vs_fp64(dev, lod, near_dist, return_dist):
     assert ev[1] > 0 and ev[0] > 0  # both passes ran
 
 
+def test_trace_mixed_normals_are_fp32_taps(dev):
+    """The mixed trace's normals are central differences of FP32-accurate
+    evaluations on the FP32 cumulative grid (the continuation pass's decoder,
+    BF16 products of exact three-term splits): against the FP64 oracle's taps
+    at the same hit points on that grid the angles stay at FP32 rounding level.
+    A BF16-level decoder or gather in those taps gives tenths of a degree."""
+    net = workloads.make_sphere_model(radius=0.6, num_lods=5, device="cpu", seed=4, noise=5e-3)
+    m = arrays(net)
+    L = 4
+    cum = oracle.cumulative_grids(m, "f32")[L][0]
+    mc = oracle.ModelArrays([cum], [m.res[L]], [m.w0[L]], [m.b0[L]], [m.w1[L]], [m.b1[L]], m.pos_invariant)
+    o, d = workloads.camera_rays(160, 120, eye=(0.4, 0.8, 2.8), device="cpu")
+    netg = net.to(dev)
+    netg.lod = L
+    rb = SphereTracer(num_steps=128, form="mixed")(netg, gpu(o.numpy(), dev), gpu(d.numpy(), dev))
+    hit = rb.hit.cpu().numpy()
+    assert hit.sum() > 2000
+    want = oracle.normals(mc, rb.x.cpu().numpy()[hit], 0, prec="f64")
+    ang = _angle_deg(rb.normal.cpu().numpy()[hit], want)
+    assert np.median(ang) < 5e-3 and np.percentile(ang, 99) < 5e-2, (np.median(ang), np.percentile(ang, 99))
+
+
 def test_trace_mixed_random_rays_inside_and_outside(dev):
     net = workloads.make_sphere_model(radius=0.45, num_lods=5, pos_invariant=False, device="cpu", seed=6)
     m = arrays(net)
