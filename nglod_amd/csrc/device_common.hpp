@@ -241,6 +241,32 @@ __device__ __forceinline__ float sdf_fp32(const ModelArgs& m, int lod, int dec, 
     return decode_fp32<POSINV>(m, dec, px, py, pz, f);
 }
 
+// Wave-wide MFMA decoders take the 64 lanes' queries as two 32-column
+// tiles.  The coordinates of the queries in this lane's column of tile 0
+// (lanes 0..31's queries) and tile 1 (lanes 32..63's): one v_permlane32_swap
+// each.
+__device__ __forceinline__ void tile_coords(float px, float py, float pz, float& qx0, float& qy0, float& qz0,
+                                            float& qx1, float& qy1, float& qz1) {
+    const auto sx = __builtin_amdgcn_permlane32_swap(__float_as_uint(px), __float_as_uint(px), false, false);
+    const auto sy = __builtin_amdgcn_permlane32_swap(__float_as_uint(py), __float_as_uint(py), false, false);
+    const auto sz = __builtin_amdgcn_permlane32_swap(__float_as_uint(pz), __float_as_uint(pz), false, false);
+    qx0 = __uint_as_float(sx[0]);
+    qx1 = __uint_as_float(sx[1]);
+    qy0 = __uint_as_float(sy[0]);
+    qy1 = __uint_as_float(sy[1]);
+    qz0 = __uint_as_float(sz[0]);
+    qz1 = __uint_as_float(sz[1]);
+}
+
+// A lane's query result from the two half-sums of its tile: part0 (tile 0)
+// on lanes 0..31 and part1 (tile 1) on lanes 32..63 hold one half each; the
+// swap brings the other half, and the sum is the same whichever operand
+// comes first.
+__device__ __forceinline__ float halves_sum(float part0, float part1) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(part0), __float_as_uint(part1), false, false);
+    return __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
+}
+
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
 // Number of set bits of `mask` below this lane.

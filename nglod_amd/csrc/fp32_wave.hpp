@@ -113,7 +113,6 @@ __device__ __forceinline__ float decode_fp32_wave(const float (&f)[kC], const Fp
     const uint64_t am = __ballot(active);
     const bool use0 = (uint32_t)am != 0u, use1 = (uint32_t)(am >> 32) != 0u;
     const uint32_t half = lane >> 5;
-    const bool lo = half == 0;
     // B operands of both tiles for channel pair kk: tile 0 takes (own channel
     // 2kk | channel 2kk+1 of lane l-32), tile 1 (channel 2kk of lane l+32 |
     // own channel 2kk+1).
@@ -125,9 +124,8 @@ __device__ __forceinline__ float decode_fp32_wave(const float (&f)[kC], const Fp
         B0[kk] = __uint_as_float(sw[0]);
         B1[kk] = __uint_as_float(sw[1]);
     }
-    const float qx0 = __shfl(px, lane & 31), qy0 = __shfl(py, lane & 31), qz0 = __shfl(pz, lane & 31);
-    const float qx1 = __shfl(px, (lane & 31) + 32), qy1 = __shfl(py, (lane & 31) + 32),
-                qz1 = __shfl(pz, (lane & 31) + 32);
+    float qx0, qy0, qz0, qx1, qy1, qz1;
+    tile_coords(px, py, pz, qx0, qy0, qz0, qx1, qy1, qz1);
     float part0 = 0.0f, part1 = 0.0f;
 #pragma unroll 1
     for (int h = 0; h < 4; ++h) {
@@ -157,9 +155,7 @@ __device__ __forceinline__ float decode_fp32_wave(const float (&f)[kC], const Fp
             part1 += layer2_tile(c, h, half, s, !POSINV, qx1, qy1, qz1);
         }
     }
-    part0 += __shfl_xor(part0, 32);
-    part1 += __shfl_xor(part1, 32);
-    return (lo ? part0 : part1) + b1;
+    return halves_sum(part0, part1) + b1;
 }
 
 // The SDF of ONE query (wave-uniform px, py, pz) computed by the whole wave,

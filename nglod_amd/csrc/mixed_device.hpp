@@ -219,38 +219,26 @@ __device__ __forceinline__ float decode_mixed_wave(const __half2 (&acc)[kC / 2],
     const uint64_t am = __ballot(active);
     const bool use0 = (uint32_t)am != 0u, use1 = (uint32_t)(am >> 32) != 0u;
     const uint32_t half = lane >> 5;
-    const bool lo = half == 0;
-    // own channels 16kb + 0..7 (Y) and 16kb + 8..15 (X) as bf16
+    // own channels 16kb + 0..7 (Y) and 16kb + 8..15 (X) as bf16; one
+    // v_permlane32_swap per word gives tile 0 (own Y | X of lane l-32) and
+    // tile 1 (Y of lane l+32 | own X)
     bf16x8 B0[2], B1[2];  // [kb] of tile 0 and tile 1
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb) {
-        uint32_t Y[4], X[4], Ys[4], Xs[4];
+        uint32_t t0[4], t1[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            Y[j] = pack_bf16(acc[8 * kb + j]);
-            X[j] = pack_bf16(acc[8 * kb + 4 + j]);
+            const auto sw = __builtin_amdgcn_permlane32_swap(pack_bf16(acc[8 * kb + j]),
+                                                             pack_bf16(acc[8 * kb + 4 + j]), false, false);
+            t0[j] = sw[0];
+            t1[j] = sw[1];
         }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            Ys[j] = __shfl_xor(Y[j], 32);
-            Xs[j] = __shfl_xor(X[j], 32);
-        }
-        uint4 t0, t1;
-        t0.x = lo ? Y[0] : Xs[0];
-        t0.y = lo ? Y[1] : Xs[1];
-        t0.z = lo ? Y[2] : Xs[2];
-        t0.w = lo ? Y[3] : Xs[3];
-        t1.x = lo ? Ys[0] : X[0];
-        t1.y = lo ? Ys[1] : X[1];
-        t1.z = lo ? Ys[2] : X[2];
-        t1.w = lo ? Ys[3] : X[3];
-        B0[kb] = __builtin_bit_cast(bf16x8, t0);
-        B1[kb] = __builtin_bit_cast(bf16x8, t1);
+        B0[kb] = __builtin_bit_cast(bf16x8, make_uint4(t0[0], t0[1], t0[2], t0[3]));
+        B1[kb] = __builtin_bit_cast(bf16x8, make_uint4(t1[0], t1[1], t1[2], t1[3]));
     }
     // query coordinates of each tile's column (tile t, column l%32 = lane l%32 + 32t)
-    const float qx0 = __shfl(px, lane & 31), qy0 = __shfl(py, lane & 31), qz0 = __shfl(pz, lane & 31);
-    const float qx1 = __shfl(px, (lane & 31) + 32), qy1 = __shfl(py, (lane & 31) + 32),
-                qz1 = __shfl(pz, (lane & 31) + 32);
+    float qx0, qy0, qz0, qx1, qy1, qz1;
+    tile_coords(px, py, pz, qx0, qy0, qz0, qx1, qy1, qz1);
     float part0 = 0.0f, part1 = 0.0f;
 #pragma unroll 1
     for (int h = 0; h < 4; ++h) {
@@ -267,9 +255,7 @@ __device__ __forceinline__ float decode_mixed_wave(const __half2 (&acc)[kC / 2],
             part1 += layer2_partial(c1, h, half, s, !POSINV, qx1, qy1, qz1);
         }
     }
-    part0 += __shfl_xor(part0, 32);
-    part1 += __shfl_xor(part1, 32);
-    return (lo ? part0 : part1) + b1;
+    return halves_sum(part0, part1) + b1;
 }
 
 template <bool POSINV, typename LV>

@@ -158,7 +158,6 @@ __device__ __forceinline__ float decode_split_wave(const float (&f)[kC], const L
     const uint64_t am_ = __ballot(active);
     const bool use0 = (uint32_t)am_ != 0u, use1 = (uint32_t)(am_ >> 32) != 0u;
     const uint32_t half = lane >> 5;
-    const bool lo = half == 0;
     // [tile][kb] B operands of the three parts
     bf16x8 Bh[2][2], Bm[2][2], Bl[2][2];
 #pragma unroll
@@ -189,9 +188,8 @@ __device__ __forceinline__ float decode_split_wave(const float (&f)[kC], const L
             Bl[t][kb] = __builtin_bit_cast(bf16x8, make_uint4(tl[t][0], tl[t][1], tl[t][2], tl[t][3]));
         }
     }
-    const float qx0 = __shfl(px, lane & 31), qy0 = __shfl(py, lane & 31), qz0 = __shfl(pz, lane & 31);
-    const float qx1 = __shfl(px, (lane & 31) + 32), qy1 = __shfl(py, (lane & 31) + 32),
-                qz1 = __shfl(pz, (lane & 31) + 32);
+    float qx0, qy0, qz0, qx1, qy1, qz1;
+    tile_coords(px, py, pz, qx0, qy0, qz0, qx1, qy1, qz1);
     float part0 = 0.0f, part1 = 0.0f;
 #pragma unroll 1
     for (int h = 0; h < 4; ++h) {
@@ -207,9 +205,7 @@ __device__ __forceinline__ float decode_split_wave(const float (&f)[kC], const L
             part1 += layer2_partial(c, h, half, s, !POSINV, qx1, qy1, qz1);
         }
     }
-    part0 += __shfl_xor(part0, 32);
-    part1 += __shfl_xor(part1, 32);
-    return (lo ? part0 : part1) + b1;
+    return halves_sum(part0, part1) + b1;
 }
 
 // Gather (gather_fp32, any level table LV) and decode of the 64 lanes'
