@@ -11,6 +11,7 @@
 
 #include "api_common.hpp"
 #include "fp32_wave.hpp"
+#include "split_wave.hpp"
 #include "trace_common.hpp"
 
 namespace nglod {
@@ -131,6 +132,50 @@ __global__ __launch_bounds__(kMarchBlock) void ao_kernel(AoArgs a) {
             const float dk = a.dist[k];
             const float q0 = p0 + n0 * dk, q1 = p1 + n1 * dk, q2 = p2 + n2 * dk;
             float r = sdf_fp32_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, q0, q1, q2, a.coarse, &coarse);
+            if (on_plane) r = fminf(r, dk);
+            const float u = (dk - r) - 0.0015f;
+            acc = acc + 3.5f * (u > 0.0f ? u : 0.0f);
+        }
+        if (valid) {
+            float v = 1.0f - acc;
+            v = v < 0.1f ? 0.1f : (v > 1.0f ? 1.0f : v);
+            a.ao[id] = v * v;
+        }
+    }
+}
+
+// The mixed form's AO: the same loop on the FP32 cumulative grid (one level,
+// no coarse levels to stage) with the decoder of the mixed trace's FP32 pass
+// (exact three-term BF16 splits on the matrix cores, FP32-level results).
+template <bool POSINV>
+__global__ __launch_bounds__(kMarchBlock) void ao_split_kernel(AoArgs a) {
+    __shared__ SplitLds lds;
+    stage_split_lds(a.m, a.dec, &lds);
+    __syncthreads();
+    const float b1 = a.m.b1[a.dec][0];
+    const uint32_t lane = lane_id();
+    bool drained = false;
+    for (;;) {
+        const int64_t id = claim_hits(a.counter, a.hit, (uint32_t)a.n, lane, drained, a.plane);
+        const bool valid = id >= 0;
+        if (__ballot(valid) == 0) break;
+        float p0 = 0.0f, p1 = 0.0f, p2 = 0.0f, n0 = 0.0f, n1 = 0.0f, n2 = 0.0f;
+        bool on_plane = false;
+        if (valid) {
+            p0 = a.x[3 * id];
+            p1 = a.x[3 * id + 1];
+            p2 = a.x[3 * id + 2];
+            n0 = a.normal[3 * id];
+            n1 = a.normal[3 * id + 1];
+            n2 = a.normal[3 * id + 2];
+            on_plane = !a.hit[id];
+        }
+        float acc = 0.0f;
+#pragma unroll 1
+        for (int k = 0; k < kAoTaps; ++k) {
+            const float dk = a.dist[k];
+            const float q0 = p0 + n0 * dk, q1 = p1 + n1 * dk, q2 = p2 + n2 * dk;
+            float r = sdf_split_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, q0, q1, q2);
             if (on_plane) r = fminf(r, dk);
             const float u = (dk - r) - 0.0015f;
             acc = acc + 3.5f * (u > 0.0f ? u : 0.0f);
@@ -302,8 +347,9 @@ extern "C" size_t nglod_ao_workspace_size(void) { return 256; }
 extern "C" int nglod_ao(const nglod_model_t* model, const float* x, const float* normal, const uint8_t* hit,
                         const uint8_t* plane_hit, int64_t n, int32_t lod, int32_t form, float* ao, void* workspace,
                         size_t workspace_bytes, void* stream) {
-    // AO taps are FP32 in both forms, as the normals are; the mixed form
-    // takes them from the FP32 cumulative grid.
+    // AO taps are FP32-level in both forms, as the normals are; the mixed
+    // form takes them from the FP32 cumulative grid with its trace's FP32-pass
+    // decoder.
     AoArgs a;
     int rc = make_model_args(model, lod, form, &a.m);
     if (rc) return rc;
@@ -330,7 +376,8 @@ extern "C" int nglod_ao(const nglod_model_t* model, const float* x, const float*
     rc = check_launch("ao_fill_kernel");
     if (rc) return rc;
     int per_cu = 0;
-    auto kern = a.m.pos_inv ? ao_kernel<true> : ao_kernel<false>;
+    auto kern = form == NGLOD_MIXED ? (a.m.pos_inv ? ao_split_kernel<true> : ao_split_kernel<false>)
+                                    : (a.m.pos_inv ? ao_kernel<true> : ao_kernel<false>);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kMarchBlock, 0) != hipSuccess || per_cu <= 0)
         per_cu = 1;
     const int64_t want = (n + kMarchBlock - 1) / kMarchBlock;

@@ -96,6 +96,29 @@ def test_ao_vs_oracle_sdf(dev):
     assert np.abs(got - want).mean() < 1e-5
 
 
+@pytest.mark.parametrize("pos_invariant", [True, False])
+def test_ao_mixed_vs_oracle_cumulative_grid(dev, pos_invariant):
+    """The mixed form's AO taps evaluate the FP32 cumulative grid with the
+    split-BF16 decoder (FP32-level): against the oracle's FP32 SDF on that
+    grid, the bar of the FP32 form's AO (per-tap differences near 1e-6)."""
+    net = workloads.make_sphere_model(radius=0.5, num_lods=4, pos_invariant=pos_invariant, seed=7,
+                                      device="cpu").to(dev)
+    net.lod = 3
+    o, d, rb = _frame(net, dev)
+    plane, _, _, _ = shadow_rays(o, d, rb, -0.45)
+    ao = ambient_occlusion(net, rb.x, rb.normal, rb.hit, plane, form="mixed")
+    torch.cuda.synchronize()
+    m = arrays(net.cpu())
+    cum = oracle.cumulative_grids(m, "f32")[3][0]
+    mc = oracle.ModelArrays([cum], [m.res[3]], [m.w0[3]], [m.b0[3]], [m.w1[3]], [m.b1[3]], m.pos_invariant)
+    want = shade.ambient_occlusion(lambda q: oracle.sdf(mc, q, 0), rb.x.cpu().numpy(), rb.normal.cpu().numpy(),
+                                   rb.hit.cpu().numpy(), plane.cpu().numpy())
+    got = ao.reshape(-1).cpu().numpy()
+    assert rb.hit.sum().item() > 500 and plane.sum().item() > 100
+    np.testing.assert_allclose(got, want, rtol=0, atol=5e-4)
+    assert np.abs(got - want).mean() < 1e-5
+
+
 @pytest.mark.parametrize("wh", [(1, 1), (5, 3), (23, 17), (1280, 720)])
 def test_shadow_map_vs_scipy(dev, wh):
     """clamp + gaussian_filter(sigma=2) equals SciPy's on the same mask: FP64
