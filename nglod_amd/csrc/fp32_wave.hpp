@@ -309,7 +309,7 @@ __device__ __forceinline__ float sdf_fp32_wave(const ModelArgs& m, int lod, cons
         if (cp.levels > 0)
             gather_fp32_coarse(m, lod, cp, cs, px, py, pz, f);
         else
-            gather_fp32(m, lod, px, py, pz, f);
+            gather_fp32_batched(m, lod, px, py, pz, f);
     } else {
 #pragma unroll
         for (int c = 0; c < kC; ++c) f[c] = 0.0f;

@@ -24,7 +24,14 @@ __global__ __launch_bounds__(256) void sdf_mixed_kernel(ModelArgs m, int lod, in
     const float py = valid ? x[3 * i + 1] : 0.0f;
     const float pz = valid ? x[3 * i + 2] : 0.0f;
     if (__ballot(valid) == 0) return;  // whole wave past the end (no barrier follows)
-    const float d = sdf_mixed_wave<POSINV>(m, lod, &lds, b1, lane, valid, px, py, pz);
+    __half2 acc[kC / 2];
+    if (valid) {
+        gather_fp16_batched(m, lod, px, py, pz, acc);
+    } else {
+#pragma unroll
+        for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
+    }
+    const float d = decode_mixed_wave<POSINV>(acc, &lds, b1, lane, valid, px, py, pz);
     if (valid) out[i] = d;
 }
 
