@@ -43,9 +43,11 @@ constexpr int kMixedBlock = 256;
 #endif
 constexpr int kPass2Block = NGLOD_MIXED_FP32_BLOCK;
 constexpr int kMixedFp32Waves = NGLOD_MIXED_FP32_WAVES;
-// Pass 1 claims rays 64 ids at a time (the FP32 march claims per idle lane):
-// measured faster for the mixed march, slightly slower for the FP32 one.
-constexpr int kPass1Chunk = 64;
+// Pass 1 claims rays 32 ids at a time (the FP32 march claims per idle lane):
+// measured faster for the mixed march, slightly slower for the FP32 one.  On
+// the 1280x720 LOD-4 frame with the claim counters one line apart: 16 ids
+// 0.725 ms, 32 0.705, 64 0.728, 128 0.858.
+constexpr int kPass1Chunk = 32;
 
 // ---------------------------------------------------------------------------
 // Cumulative grid of level li: at corner (cx, cy, cz) of grid li, the sum over
