@@ -183,7 +183,7 @@ int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const floa
  * hipMalloc): nglod_trace_dense without a workspace argument; the same
  * kernels and the same outputs, bit for bit.  Its scratch (claim counters,
  * and for NGLOD_MIXED the hand-off records and flags) is a file-scope device
- * array of 256 B + 49 B * NGLOD_NOALLOC_MAX_MIXED_RAYS (about 49 MiB) that each
+ * array of 3 KiB + 49 B * NGLOD_NOALLOC_MAX_MIXED_RAYS (about 49 MiB) that each
  * device holds from the library's load until the process ends.  The call
  * allocates nothing.  Limits: NGLOD_MIXED takes n <= NGLOD_NOALLOC_MAX_MIXED_RAYS
  * (larger n: NGLOD_ERR_CAPACITY; rays are independent, so a larger frame can
