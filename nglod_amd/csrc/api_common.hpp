@@ -133,8 +133,9 @@ class PassTimer {
     int n_ = 0;
 };
 
-// Slots of nglod_trace_params_t::pass_ms.
-enum : int { kPassMarch = 0, kPassFp32 = 1, kPassNormals = 2, kPassOther = 3, kPassSlots = 4 };
+// Slots of nglod_trace_params_t::pass_ms.  Slot 2 (a separate normals pass)
+// stays 0: the mixed form's FP32 pass takes the normal taps.
+enum : int { kPassMarch = 0, kPassFp32 = 1, kPassUnused = 2, kPassOther = 3, kPassSlots = 4 };
 
 struct TraceArgs;
 // Mixed-precision launchers (mixed.hip).

@@ -317,20 +317,4 @@ __device__ __forceinline__ float sdf_fp32_wave(const ModelArgs& m, int lod, cons
     return decode_fp32_wave<POSINV>(f, s, b1, lane, active, px, py, pz);
 }
 
-// sdf_fp32_wave with the grid pointers and resolutions from a level table
-// (any type with res[] and grid[]) and no coarse levels in LDS: the same
-// bits.
-template <bool POSINV, typename LV>
-__device__ __forceinline__ float sdf_fp32_wave_lv(const LV& lv, int lod, const Fp32Lds* s, float b1,
-                                                  uint32_t lane, bool active, float px, float py, float pz) {
-    float f[kC];
-    if (active) {
-        gather_fp32(lv, lod, px, py, pz, f);
-    } else {
-#pragma unroll
-        for (int c = 0; c < kC; ++c) f[c] = 0.0f;
-    }
-    return decode_fp32_wave<POSINV>(f, s, b1, lane, active, px, py, pz);
-}
-
 }  // namespace nglod
