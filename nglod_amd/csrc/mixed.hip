@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
         const bool busy = s.phase != PH_IDLE;
         __half2 acc[kC / 2];
         gather_fp16_coop(&levels, a.lod, lane, act, s.qx, s.qy, s.qz, acc);
-        const float d = decode_mixed_wave<POSINV>(acc, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        const float d = decode_mixed_wave_few<POSINV>(acc, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
             hist.push(s);
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void trace_mixed_pass
         const bool busy = s.phase != PH_IDLE;
         float f[kC];
         gather_fp32_coop(&solo_lv, a.lod, lane, act, s.qx, s.qy, s.qz, f);
-        const float d = decode_split_wave<POSINV>(f, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        const float d = decode_split_wave_few<POSINV>(f, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
             s.consume(a, d);
@@ -573,7 +573,7 @@ __global__ __launch_bounds__(kMixedBlock, 3) void pool_pass1_kernel(TraceArgs a,
             hold_request(p, g, held, sl, lane, stage_pool_mixed);
             __half2 acc[kC / 2];
             gather_fp16_coop(&sl->lv, p.req[g].lod, lane, gm, s.qx, s.qy, s.qz, acc);
-            const float dg = decode_mixed_wave<POSINV>(acc, &sl->dec, sl->b1, lane, in, s.qx, s.qy, s.qz);
+            const float dg = decode_mixed_wave_few<POSINV>(acc, &sl->dec, sl->b1, lane, in, s.qx, s.qy, s.qz);
             if (in) d = dg;
         }
         if (busy) {
@@ -697,7 +697,7 @@ __global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a,
             hold_request(p, g, held, sl, lane, stage_pool_fp32);
             float f[kC];
             gather_fp32_coop(&sl->lv, p.req[g].lod, lane, gm, s.qx, s.qy, s.qz, f);
-            const float dg = decode_split_wave<POSINV>(f, &sl->dec, sl->b1, lane, in, s.qx, s.qy, s.qz);
+            const float dg = decode_split_wave_few<POSINV>(f, &sl->dec, sl->b1, lane, in, s.qx, s.qy, s.qz);
             if (in) d = dg;
         }
         if (s.phase != PH_IDLE) {

@@ -258,6 +258,53 @@ __device__ __forceinline__ float decode_mixed_wave(const __half2 (&acc)[kC / 2],
     return halves_sum(part0, part1) + b1;
 }
 
+// decode_mixed_wave for a wave with few busy lanes, where the decode's
+// latency rather than its issue rate sets the trip time: a used tile's four
+// hidden tiles are independent, so their LDS reads, MFMA pairs and
+// second-layer chains are issued together instead of one hidden tile after
+// another.  Per output the same instructions on the same operands, and the
+// partial sums added in the same order: the same bits as decode_mixed_wave.
+template <bool POSINV>
+__device__ __forceinline__ float decode_mixed_wave_few(const __half2 (&acc)[kC / 2], const MixedLds* s, float b1,
+                                                       uint32_t lane, bool active, float px, float py, float pz) {
+    const uint64_t am = __ballot(active);
+    const uint32_t half = lane >> 5;
+    bf16x8 B[2][2];  // [tile][kb]
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        uint32_t t0[4], t1[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const auto sw = __builtin_amdgcn_permlane32_swap(pack_bf16(acc[8 * kb + j]),
+                                                             pack_bf16(acc[8 * kb + 4 + j]), false, false);
+            t0[j] = sw[0];
+            t1[j] = sw[1];
+        }
+        B[0][kb] = __builtin_bit_cast(bf16x8, make_uint4(t0[0], t0[1], t0[2], t0[3]));
+        B[1][kb] = __builtin_bit_cast(bf16x8, make_uint4(t1[0], t1[1], t1[2], t1[3]));
+    }
+    float qx0, qy0, qz0, qx1, qy1, qz1;
+    tile_coords(px, py, pz, qx0, qy0, qz0, qx1, qy1, qz1);
+    float part[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        if ((t == 0 ? (uint32_t)am : (uint32_t)(am >> 32)) == 0u) continue;
+        float p[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const bf16x8 a0 = __builtin_bit_cast(bf16x8, s->wfrag[(h * 2 + 0) * 64 + lane]);
+            const bf16x8 a1 = __builtin_bit_cast(bf16x8, s->wfrag[(h * 2 + 1) * 64 + lane]);
+            f32x16 c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, B[t][0], bias_tile(h, half, s), 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, B[t][1], c, 0, 0, 0);
+            p[h] = layer2_partial(c, h, half, s, !POSINV, t == 0 ? qx0 : qx1, t == 0 ? qy0 : qy1,
+                                  t == 0 ? qz0 : qz1);
+        }
+#pragma unroll
+        for (int h = 0; h < 4; ++h) part[t] += p[h];
+    }
+    return halves_sum(part[0], part[1]) + b1;
+}
+
 template <bool POSINV, typename LV>
 __device__ __forceinline__ float sdf_mixed_wave(const LV& m, int lod, const MixedLds* s, float b1,
                                                 uint32_t lane, bool active, float px, float py, float pz) {

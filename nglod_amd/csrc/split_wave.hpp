@@ -208,6 +208,60 @@ __device__ __forceinline__ float decode_split_wave(const float (&f)[kC], const L
     return halves_sum(part0, part1) + b1;
 }
 
+// decode_split_wave for a wave with few busy lanes (latency-bound): a used
+// tile's four hidden tiles are independent and issued together instead of
+// one after another.  Per output the same MFMA chain on the same operands and
+// the partial sums added in the same order: the same bits as
+// decode_split_wave.
+template <bool POSINV, typename LDS>
+__device__ __forceinline__ float decode_split_wave_few(const float (&f)[kC], const LDS* s, float b1, uint32_t lane,
+                                                       bool active, float px, float py, float pz) {
+    const uint64_t am_ = __ballot(active);
+    const uint32_t half = lane >> 5;
+    float qx0, qy0, qz0, qx1, qy1, qz1;
+    tile_coords(px, py, pz, qx0, qy0, qz0, qx1, qy1, qz1);
+    float part[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        if ((t == 0 ? (uint32_t)am_ : (uint32_t)(am_ >> 32)) == 0u) continue;
+        bf16x8 Bh[2], Bm[2], Bl[2];  // [kb] of tile t
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb) {
+            uint32_t th[4], tm[4], tl[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                __bf16 h[2], m[2], l[2];
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    const int kk = 8 * kb + 2 * q + p;
+                    const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(f[2 * kk]),
+                                                                     __float_as_uint(f[2 * kk + 1]), false, false);
+                    split3(__uint_as_float(sw[t]), h[p], m[p], l[p]);
+                }
+                th[q] = bf16_pair(h[0], h[1]);
+                tm[q] = bf16_pair(m[0], m[1]);
+                tl[q] = bf16_pair(l[0], l[1]);
+            }
+            Bh[kb] = __builtin_bit_cast(bf16x8, make_uint4(th[0], th[1], th[2], th[3]));
+            Bm[kb] = __builtin_bit_cast(bf16x8, make_uint4(tm[0], tm[1], tm[2], tm[3]));
+            Bl[kb] = __builtin_bit_cast(bf16x8, make_uint4(tl[0], tl[1], tl[2], tl[3]));
+        }
+        float p[4];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            bf16x8 ah[2], am[2], al[2];
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb) split_afrag(s, h, kb, lane, ah[kb], am[kb], al[kb]);
+            const f32x16 c = split_tile(ah, am, al, Bh, Bm, Bl, bias_tile(h, half, s));
+            p[h] = layer2_partial(c, h, half, s, !POSINV, t == 0 ? qx0 : qx1, t == 0 ? qy0 : qy1,
+                                  t == 0 ? qz0 : qz1);
+        }
+#pragma unroll
+        for (int h = 0; h < 4; ++h) part[t] += p[h];
+    }
+    return halves_sum(part[0], part[1]) + b1;
+}
+
 // Gather (gather_fp32, any level table LV) and decode of the 64 lanes'
 // queries.  Wave-wide.
 template <bool POSINV, typename LV, typename LDS>
