@@ -158,18 +158,19 @@ __device__ __forceinline__ void gather_fp32(const LV& m, int lod, float px, floa
     }
 }
 
-// gather_fp32 with the corner loads issued in two batches of four corners
-// (32 16-byte loads each) ahead of their FMAs, so that a wave waits about two
-// memory latencies per level instead of one per handful of loads (a wave-wide
-// load completes with its slowest lane, and on an L1 miss that is an L2 round
-// trip).  Same corners, weights and FMA order as gather_fp32: the same bits.
-// For kernels with registers to spare (128 VGPRs hold a batch).
-template <typename LV>
-__device__ __forceinline__ void gather_fp32_batched(const LV& m, int lod, float px, float py, float pz,
-                                                    float (&f)[kC]) {
-#pragma unroll
-    for (int c = 0; c < kC; ++c) f[c] = 0.0f;
-    for (int l = 0; l <= lod; ++l) {
+// gather_fp32's accumulation of levels 0..lod onto f (NL levels when NL > 0
+// fixes the count at compile time) with the corner loads issued in two
+// batches of four corners (32 16-byte loads each) ahead of their FMAs, so
+// that a wave waits about two memory latencies per level instead of one per
+// handful of loads (a wave-wide load completes with its slowest lane, and on
+// an L1 miss that is an L2 round trip).  Same corners, weights and FMA order
+// as gather_fp32: the same bits.  For kernels with registers to spare (128
+// VGPRs hold a batch).
+template <int NL = 0, typename LV>
+__device__ __forceinline__ void gather_fp32_add(const LV& m, int lod, float px, float py, float pz,
+                                                float (&f)[kC]) {
+    const int nl = NL > 0 ? NL : lod + 1;
+    for (int l = 0; l < nl; ++l) {
         const int r = m.res[l];
         const int S = r + 1;
         const AxisW ax = axis_weights(px, r);
@@ -204,6 +205,26 @@ __device__ __forceinline__ void gather_fp32_batched(const LV& m, int lod, float 
             __builtin_amdgcn_sched_barrier(0);
         }
     }
+}
+
+// gather_fp32 (levels 0..lod) through gather_fp32_add.
+template <typename LV>
+__device__ __forceinline__ void gather_fp32_batched(const LV& m, int lod, float px, float py, float pz,
+                                                    float (&f)[kC]) {
+#pragma unroll
+    for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+    gather_fp32_add(m, lod, px, py, pz, f);
+}
+
+// The mixed form's FP32 gather: one level, the cumulative grid (the host
+// launches check lod 0), zero on inactive lanes, the accumulators cleared
+// once for both cases.  The same bits as gather_fp32(m, 0, ...).
+template <typename LV>
+__device__ __forceinline__ void gather_fp32_cum(const LV& m, bool active, float px, float py, float pz,
+                                                float (&f)[kC]) {
+#pragma unroll
+    for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+    if (active) gather_fp32_add<1>(m, 0, px, py, pz, f);
 }
 
 // relu(e W0^T + b0) W1^T + b1 with e = [xyz, f] (POSINV=false) or f

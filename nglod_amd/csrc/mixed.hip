@@ -248,12 +248,7 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
         }
         const bool busy = s.phase != PH_IDLE;
         __half2 acc[kC / 2];
-        if (busy) {
-            gather_fp16_batched(a.m, a.lod, s.qx, s.qy, s.qz, acc);
-        } else {
-#pragma unroll
-            for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
-        }
+        gather_fp16_cum(a.m, busy, s.qx, s.qy, s.qz, acc);
         const float d = decode_mixed_wave<POSINV>(acc, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
@@ -273,7 +268,7 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
         if (act == 0) break;
         const bool busy = s.phase != PH_IDLE;
         __half2 acc[kC / 2];
-        gather_fp16_coop(&levels, a.lod, lane, act, s.qx, s.qy, s.qz, acc);
+        gather_fp16_coop(&levels, lane, act, s.qx, s.qy, s.qz, acc);
         const float d = decode_mixed_wave_few<POSINV>(acc, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
@@ -317,7 +312,7 @@ __global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void trace_mixed_pass
             break;
         }
         const bool busy = s.phase != PH_IDLE;
-        const float d = sdf_split_wave<POSINV>(a.m, a.lod, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
+        const float d = sdf_split_wave<POSINV>(a.m, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
             s.consume(a, d);
@@ -334,7 +329,7 @@ __global__ __launch_bounds__(kPass2Block, kMixedFp32Waves) void trace_mixed_pass
         if (act == 0) break;
         const bool busy = s.phase != PH_IDLE;
         float f[kC];
-        gather_fp32_coop(&solo_lv, a.lod, lane, act, s.qx, s.qy, s.qz, f);
+        gather_fp32_coop(&solo_lv, lane, act, s.qx, s.qy, s.qz, f);
         const float d = decode_split_wave_few<POSINV>(f, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;
@@ -572,7 +567,7 @@ __global__ __launch_bounds__(kMixedBlock, 3) void pool_pass1_kernel(TraceArgs a,
             todo &= ~gm;
             hold_request(p, g, held, sl, lane, stage_pool_mixed);
             __half2 acc[kC / 2];
-            gather_fp16_coop(&sl->lv, p.req[g].lod, lane, gm, s.qx, s.qy, s.qz, acc);
+            gather_fp16_coop(&sl->lv, lane, gm, s.qx, s.qy, s.qz, acc);
             const float dg = decode_mixed_wave_few<POSINV>(acc, &sl->dec, sl->b1, lane, in, s.qx, s.qy, s.qz);
             if (in) d = dg;
         }
@@ -672,7 +667,7 @@ __global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a,
             const bool in = busy && rq == g;
             todo &= ~__ballot(in);
             hold_request(p, g, held, sl, lane, stage_pool_fp32);
-            const float dg = sdf_split_wave<POSINV>(pool_levels32(p, g), p.req[g].lod, &sl->dec, sl->b1, lane, in,
+            const float dg = sdf_split_wave<POSINV>(pool_levels32(p, g), &sl->dec, sl->b1, lane, in,
                                                     s.qx, s.qy, s.qz);
             if (in) d = dg;
         }
@@ -696,7 +691,7 @@ __global__ __launch_bounds__(kPoolFp32Block) void pool_pass2_kernel(TraceArgs a,
             todo &= ~gm;
             hold_request(p, g, held, sl, lane, stage_pool_fp32);
             float f[kC];
-            gather_fp32_coop(&sl->lv, p.req[g].lod, lane, gm, s.qx, s.qy, s.qz, f);
+            gather_fp32_coop(&sl->lv, lane, gm, s.qx, s.qy, s.qz, f);
             const float dg = decode_split_wave_few<POSINV>(f, &sl->dec, sl->b1, lane, in, s.qx, s.qy, s.qz);
             if (in) d = dg;
         }
@@ -794,6 +789,8 @@ static int trace_mixed_rounds(TraceArgs a, int64_t n, hipStream_t s, PassTimer& 
 }
 
 int launch_trace_mixed(const TraceArgs& a, int64_t n, hipStream_t s, PassTimer& timer) {
+    // the mixed kernels gather one level, the cumulative grid of the LOD
+    if (a.lod != 0 || a.m.num_lods != 1) return fail(NGLOD_ERR_ARG, "mixed form: one cumulative level (gather lod 0)");
     return a.m.pos_inv ? trace_mixed_rounds<true>(a, n, s, timer) : trace_mixed_rounds<false>(a, n, s, timer);
 }
 

@@ -95,16 +95,16 @@ __device__ __forceinline__ void gather_fp16(const LV& m, int lod, float px, floa
     }
 }
 
-// gather_fp16 with the corner loads issued in two batches of four corners
-// (16 16-byte loads each) ahead of their FMAs: about two memory latencies per
-// level instead of one per handful of loads.  Same corners, weights and FMA
-// order as gather_fp16: the same bits.
-template <typename LV>
-__device__ __forceinline__ void gather_fp16_batched(const LV& m, int lod, float px, float py, float pz,
-                                                    __half2 (&acc)[kC / 2]) {
-#pragma unroll
-    for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
-    for (int l = 0; l <= lod; ++l) {
+// gather_fp16's accumulation of levels 0..lod onto acc (NL levels when NL >
+// 0 fixes the count at compile time) with the corner loads issued in two
+// batches of four corners (16 16-byte loads each) ahead of their FMAs: about
+// two memory latencies per level instead of one per handful of loads.  Same
+// corners, weights and FMA order as gather_fp16: the same bits.
+template <int NL = 0, typename LV>
+__device__ __forceinline__ void gather_fp16_add(const LV& m, int lod, float px, float py, float pz,
+                                                __half2 (&acc)[kC / 2]) {
+    const int nl = NL > 0 ? NL : lod + 1;
+    for (int l = 0; l < nl; ++l) {
         const int r = m.res[l];
         const int S = r + 1;
         const AxisW ax = axis_weights(px, r);
@@ -141,6 +141,19 @@ __device__ __forceinline__ void gather_fp16_batched(const LV& m, int lod, float 
             __builtin_amdgcn_sched_barrier(0);
         }
     }
+}
+
+// The mixed form's FP16 gather: its ModelArgs hold one level, the cumulative
+// grid (the host launches check lod 0), so the level loop is gone; zero on
+// inactive lanes.  The accumulators are cleared once for both cases (a clear
+// on each side of the branch costs both on every trip).  The same bits as
+// gather_fp16(m, 0, ...).
+template <typename LV>
+__device__ __forceinline__ void gather_fp16_cum(const LV& m, bool active, float px, float py, float pz,
+                                                __half2 (&acc)[kC / 2]) {
+#pragma unroll
+    for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
+    if (active) gather_fp16_add<1>(m, 0, px, py, pz, acc);
 }
 
 __device__ __forceinline__ uint32_t pack_bf16(__half2 h) {
@@ -337,25 +350,24 @@ __device__ __forceinline__ void stage_mixed_levels(const ModelArgs& m, MixedLeve
     }
 }
 
-// gather_fp16 for the busy lanes of a wave with at most 4 of them (act, a
-// wave-uniform mask), done by 16 lanes per query: lane 16 i + j computes
-// channel pair j of the i-th busy lane's query with every level's 8 corner
-// loads issued first (one round of memory latency instead of one per level),
-// then the same __hfma2 chain in the same level / corner order; each busy
-// lane then collects its 16 pairs.  Bit-identical to gather_fp16.  Every lane
-// of the wave must call this.
-__device__ __forceinline__ void gather_fp16_coop(const MixedLevels* lv, int lod, uint32_t lane, uint64_t act,
-                                                 float qx, float qy, float qz, __half2 (&acc)[kC / 2]) {
+// gather_fp16_cum for the busy lanes of a wave with at most 4 of them (act,
+// a wave-uniform mask), done by 16 lanes per query: lane 16 i + j computes
+// channel pair j of the i-th busy lane's query with the 8 corner loads of the
+// cumulative grid (level 0) issued first, then the same __hfma2 chain in the
+// same corner order; each busy lane then collects its 16 pairs.
+// Bit-identical to gather_fp16_cum.  Every lane of the wave must call this.
+__device__ __forceinline__ void gather_fp16_coop(const MixedLevels* lv, uint32_t lane, uint64_t act, float qx,
+                                                 float qy, float qz, __half2 (&acc)[kC / 2]) {
+    constexpr int lod = 0;
     const int i = (int)(lane >> 4), j = (int)(lane & 15);
     const int k = __popcll(act);
     const bool helper = i < k;
     const int src = helper ? nth_set_bit(act, i) : 0;
     const float px = __shfl(qx, src), py = __shfl(qy, src), pz = __shfl(qz, src);
-    uint32_t g[NGLOD_MAX_LODS][8];
+    uint32_t g[lod + 1][8];
     if (helper) {
 #pragma unroll
-        for (int l = 0; l < NGLOD_MAX_LODS; ++l) {
-            if (l > lod) break;
+        for (int l = 0; l <= lod; ++l) {
             const int r = lv->res[l];
             const int S = r + 1;
             const AxisW ax = axis_weights(px, r);
@@ -377,8 +389,7 @@ __device__ __forceinline__ void gather_fp16_coop(const MixedLevels* lv, int lod,
     __half2 v = __float2half2_rn(0.0f);
     if (helper) {
 #pragma unroll
-        for (int l = 0; l < NGLOD_MAX_LODS; ++l) {
-            if (l > lod) break;
+        for (int l = 0; l <= lod; ++l) {
             const int r = lv->res[l];
             const AxisW ax = axis_weights(px, r);
             const AxisW ay = axis_weights(py, r);

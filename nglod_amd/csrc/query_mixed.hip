@@ -9,7 +9,7 @@
 namespace nglod {
 
 template <bool POSINV>
-__global__ __launch_bounds__(256) void sdf_mixed_kernel(ModelArgs m, int lod, int dec, const float* __restrict__ x,
+__global__ __launch_bounds__(256) void sdf_mixed_kernel(ModelArgs m, int dec, const float* __restrict__ x,
                                                         int64_t n, float* __restrict__ out,
                                                         const uint32_t* __restrict__ order) {
     __shared__ MixedLds lds;
@@ -25,24 +25,21 @@ __global__ __launch_bounds__(256) void sdf_mixed_kernel(ModelArgs m, int lod, in
     const float pz = valid ? x[3 * i + 2] : 0.0f;
     if (__ballot(valid) == 0) return;  // whole wave past the end (no barrier follows)
     __half2 acc[kC / 2];
-    if (valid) {
-        gather_fp16_batched(m, lod, px, py, pz, acc);
-    } else {
-#pragma unroll
-        for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
-    }
+    gather_fp16_cum(m, valid, px, py, pz, acc);
     const float d = decode_mixed_wave<POSINV>(acc, &lds, b1, lane, valid, px, py, pz);
     if (valid) out[i] = d;
 }
 
 int launch_sdf_mixed(const ModelArgs& m, int lod, int dec, const float* x, int64_t n, float* out,
                      const uint32_t* order, hipStream_t s) {
+    // a mixed-form ModelArgs holds one level, the cumulative grid of the LOD
+    if (lod != 0 || m.num_lods != 1) return fail(NGLOD_ERR_ARG, "mixed form: one cumulative level (gather lod 0)");
     const int64_t blocks = (n + 255) / 256;
     if (m.pos_inv)
-        hipLaunchKernelGGL(sdf_mixed_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, m, lod, dec, x, n, out,
+        hipLaunchKernelGGL(sdf_mixed_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, m, dec, x, n, out,
                            order);
     else
-        hipLaunchKernelGGL(sdf_mixed_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, m, lod, dec, x, n, out,
+        hipLaunchKernelGGL(sdf_mixed_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, m, dec, x, n, out,
                            order);
     return check_launch("sdf_mixed_kernel");
 }

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kMarchBlock) void ao_split_kernel(AoArgs a) {
         for (int k = 0; k < kAoTaps; ++k) {
             const float dk = a.dist[k];
             const float q0 = p0 + n0 * dk, q1 = p1 + n1 * dk, q2 = p2 + n2 * dk;
-            float r = sdf_split_wave<POSINV>(a.m, a.lod, &lds, b1, lane, valid, q0, q1, q2);
+            float r = sdf_split_wave<POSINV>(a.m, &lds, b1, lane, valid, q0, q1, q2);
             if (on_plane) r = fminf(r, dk);
             const float u = (dk - r) - 0.0015f;
             acc = acc + 3.5f * (u > 0.0f ? u : 0.0f);

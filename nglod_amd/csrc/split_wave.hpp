@@ -262,41 +262,35 @@ __device__ __forceinline__ float decode_split_wave_few(const float (&f)[kC], con
     return halves_sum(part[0], part[1]) + b1;
 }
 
-// Gather (gather_fp32, any level table LV) and decode of the 64 lanes'
-// queries.  Wave-wide.
+// Gather of the cumulative grid (gather_fp32_cum, any level table LV whose
+// level 0 is that grid) and decode of the 64 lanes' queries.  Wave-wide.
 template <bool POSINV, typename LV, typename LDS>
-__device__ __forceinline__ float sdf_split_wave(const LV& lv, int lod, const LDS* s, float b1, uint32_t lane,
-                                                bool active, float px, float py, float pz) {
+__device__ __forceinline__ float sdf_split_wave(const LV& lv, const LDS* s, float b1, uint32_t lane, bool active,
+                                                float px, float py, float pz) {
     float f[kC];
-    if (active) {
-        gather_fp32_batched(lv, lod, px, py, pz, f);
-    } else {
-#pragma unroll
-        for (int c = 0; c < kC; ++c) f[c] = 0.0f;
-    }
+    gather_fp32_cum(lv, active, px, py, pz, f);
     return decode_split_wave<POSINV, LDS>(f, s, b1, lane, active, px, py, pz);
 }
 
-// gather_fp32 for the busy lanes of a wave with at most 4 of them (act, a
-// wave-uniform mask), done by 16 lanes per query: lane 16 i + j computes
-// channels 2j and 2j + 1 of the i-th busy lane's query with every level's 8
-// corner loads issued first (one round of memory latency instead of one per
-// level), then the same FMAs in the same level / corner order as
-// gather_fp32; each busy lane then collects its 32 channels.  Bit-identical
-// to gather_fp32.  Every lane of the wave must call this.
+// gather_fp32_cum for the busy lanes of a wave with at most 4 of them (act,
+// a wave-uniform mask), done by 16 lanes per query: lane 16 i + j computes
+// channels 2j and 2j + 1 of the i-th busy lane's query with the 8 corner
+// loads of the cumulative grid (level 0) issued first, then the same FMAs in
+// the same corner order; each busy lane then collects its 32 channels.
+// Bit-identical to gather_fp32_cum.  Every lane of the wave must call this.
 template <typename LV>
-__device__ __forceinline__ void gather_fp32_coop(const LV* lv, int lod, uint32_t lane, uint64_t act, float qx,
-                                                 float qy, float qz, float (&f)[kC]) {
+__device__ __forceinline__ void gather_fp32_coop(const LV* lv, uint32_t lane, uint64_t act, float qx, float qy,
+                                                 float qz, float (&f)[kC]) {
+    constexpr int lod = 0;
     const int i = (int)(lane >> 4), j = (int)(lane & 15);
     const int k = __popcll(act);
     const bool helper = i < k;
     const int src = helper ? nth_set_bit(act, i) : 0;
     const float px = __shfl(qx, src), py = __shfl(qy, src), pz = __shfl(qz, src);
-    float2 g[NGLOD_MAX_LODS][8];
+    float2 g[lod + 1][8];
     if (helper) {
 #pragma unroll
-        for (int l = 0; l < NGLOD_MAX_LODS; ++l) {
-            if (l > lod) break;
+        for (int l = 0; l <= lod; ++l) {
             const int r = lv->res[l];
             const int S = r + 1;
             const AxisW ax = axis_weights(px, r);
@@ -318,8 +312,7 @@ __device__ __forceinline__ void gather_fp32_coop(const LV* lv, int lod, uint32_t
     float v0 = 0.0f, v1 = 0.0f;
     if (helper) {
 #pragma unroll
-        for (int l = 0; l < NGLOD_MAX_LODS; ++l) {
-            if (l > lod) break;
+        for (int l = 0; l <= lod; ++l) {
             const int r = lv->res[l];
             const AxisW ax = axis_weights(px, r);
             const AxisW ay = axis_weights(py, r);
