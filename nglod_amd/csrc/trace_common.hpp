@@ -501,6 +501,7 @@ struct RayFeed {
     // and its lane takes the next one.
     template <int CHUNK>
     __device__ __forceinline__ void refill(RayState& s, const TraceArgs& a, uint32_t lane) {
+        if (__ballot(s.phase == PH_IDLE && !empty()) == 0) return;
         for (;;) {
             const bool need = s.phase == PH_IDLE;
             if (__ballot(need && !empty()) == 0) break;
