@@ -216,9 +216,10 @@ __device__ __forceinline__ void gather_fp32_batched(const LV& m, int lod, float 
     gather_fp32_add(m, lod, px, py, pz, f);
 }
 
-// The mixed form's FP32 gather: one level, the cumulative grid (the host
-// launches check lod 0), zero on inactive lanes, the accumulators cleared
-// once for both cases.  The same bits as gather_fp32(m, 0, ...).
+// The mixed form's FP32 gather: one level, the cumulative grid of the LOD (a
+// mixed-form ModelArgs holds only that level, gathered as lod 0), zero on
+// inactive lanes, the accumulators cleared once for both cases.  The same
+// bits as gather_fp32(m, 0, ...).
 template <typename LV>
 __device__ __forceinline__ void gather_fp32_cum(const LV& m, bool active, float px, float py, float pz,
                                                 float (&f)[kC]) {

@@ -143,9 +143,9 @@ __device__ __forceinline__ void gather_fp16_add(const LV& m, int lod, float px, 
     }
 }
 
-// The mixed form's FP16 gather: its ModelArgs hold one level, the cumulative
-// grid (the host launches check lod 0), so the level loop is gone; zero on
-// inactive lanes.  The accumulators are cleared once for both cases (a clear
+// The mixed form's FP16 gather: a mixed-form ModelArgs (make_model_args)
+// holds one level, the cumulative grid of the LOD, which every mixed launch
+// gathers as lod 0, so the level loop is gone; zero on inactive lanes.  The accumulators are cleared once for both cases (a clear
 // on each side of the branch costs both on every trip).  The same bits as
 // gather_fp16(m, 0, ...).
 template <typename LV>

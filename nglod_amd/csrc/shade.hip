@@ -361,6 +361,9 @@ extern "C" int nglod_ao(const nglod_model_t* model, const float* x, const float*
         return fail(NGLOD_ERR_WORKSPACE, "workspace smaller than nglod_ao_workspace_size()");
     a.lod = gather_lod(form, lod);
     a.dec = decoder_of(model, lod);
+    // the mixed AO kernel gathers one level, the cumulative grid of the LOD
+    if (form == NGLOD_MIXED && (a.lod != 0 || a.m.num_lods != 1))
+        return fail(NGLOD_ERR_ARG, "mixed form: one cumulative level (gather lod 0)");
     a.coarse = coarse_plan(a.m, a.lod);
     a.x = x;
     a.normal = normal;
