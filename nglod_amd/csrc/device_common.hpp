@@ -218,8 +218,7 @@ __device__ __forceinline__ void gather_fp32_batched(const LV& m, int lod, float 
 
 // The mixed form's FP32 gather: one level, the cumulative grid of the LOD (a
 // mixed-form ModelArgs holds only that level, gathered as lod 0), zero on
-// inactive lanes, the accumulators cleared once for both cases.  The same
-// bits as gather_fp32(m, 0, ...).
+// inactive lanes, the accumulators cleared once for both cases.
 template <typename LV>
 __device__ __forceinline__ void gather_fp32_cum(const LV& m, bool active, float px, float py, float pz,
                                                 float (&f)[kC]) {

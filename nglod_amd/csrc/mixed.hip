@@ -390,7 +390,7 @@ struct ScalarPtrs {
     const T* const* p;
     __device__ __forceinline__ const T* operator[](int i) const { return ((const T* const NGLOD_CONST*)p)[i]; }
 };
-struct PoolLevelsH {  // FP16 grids (gather_fp16)
+struct PoolLevelsH {  // FP16 grids (gather_fp16_cum)
     ScalarInts res;
     ScalarPtrs<uint16_t> grid_h;
 };
@@ -539,7 +539,7 @@ __global__ __launch_bounds__(kMixedBlock, 3) void pool_pass1_kernel(TraceArgs a,
             const bool in = busy && rq == g;
             todo &= ~__ballot(in);
             hold_request(p, g, held, sl, lane, stage_pool_mixed);
-            const float dg = sdf_mixed_wave<POSINV>(pool_levels_h(p, g), p.req[g].lod, &sl->dec, sl->b1, lane, in,
+            const float dg = sdf_mixed_wave<POSINV>(pool_levels_h(p, g), &sl->dec, sl->b1, lane, in,
                                                     s.qx, s.qy, s.qz);
             if (in) d = dg;
         }

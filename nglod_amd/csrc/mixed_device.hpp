@@ -53,53 +53,14 @@ __device__ __forceinline__ void stage_mixed_lds(const ModelArgs& m, int dec, Mix
     }
 }
 
-// Accumulates w * grid_h[corner][0..31] into acc (16 packed FP16 pairs).
-__device__ __forceinline__ void corner_hfma(__half2 (&acc)[kC / 2], const __half* __restrict__ p, __half2 w) {
-    const uint4* q = reinterpret_cast<const uint4*>(p);
-#pragma unroll
-    for (int v = 0; v < kC / 8; ++v) {
-        const uint4 g = q[v];
-        acc[4 * v + 0] = __hfma2(__builtin_bit_cast(__half2, g.x), w, acc[4 * v + 0]);
-        acc[4 * v + 1] = __hfma2(__builtin_bit_cast(__half2, g.y), w, acc[4 * v + 1]);
-        acc[4 * v + 2] = __hfma2(__builtin_bit_cast(__half2, g.z), w, acc[4 * v + 2]);
-        acc[4 * v + 3] = __hfma2(__builtin_bit_cast(__half2, g.w), w, acc[4 * v + 3]);
-    }
-}
-
-// LV: any table with res[] and grid_h[] per level (ModelArgs, or
-// MixedLevels in LDS).
-template <typename LV>
-__device__ __forceinline__ void gather_fp16(const LV& m, int lod, float px, float py, float pz,
-                                            __half2 (&acc)[kC / 2]) {
-#pragma unroll
-    for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
-    for (int l = 0; l <= lod; ++l) {
-        const int r = m.res[l];
-        const int S = r + 1;
-        const AxisW ax = axis_weights(px, r);
-        const AxisW ay = axis_weights(py, r);
-        const AxisW az = axis_weights(pz, r);
-        const __half* base = reinterpret_cast<const __half*>(m.grid_h[l]) +
-                             ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
-        const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
-        const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
-        const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
-        corner_hfma(acc, base, __float2half2_rn(wxy00 * az.w0));
-        corner_hfma(acc, base + sx, __float2half2_rn(wxy10 * az.w0));
-        corner_hfma(acc, base + sy, __float2half2_rn(wxy01 * az.w0));
-        corner_hfma(acc, base + sy + sx, __float2half2_rn(wxy11 * az.w0));
-        corner_hfma(acc, base + sz, __float2half2_rn(wxy00 * az.w1));
-        corner_hfma(acc, base + sz + sx, __float2half2_rn(wxy10 * az.w1));
-        corner_hfma(acc, base + sz + sy, __float2half2_rn(wxy01 * az.w1));
-        corner_hfma(acc, base + sz + sy + sx, __float2half2_rn(wxy11 * az.w1));
-    }
-}
-
-// gather_fp16's accumulation of levels 0..lod onto acc (NL levels when NL >
-// 0 fixes the count at compile time) with the corner loads issued in two
-// batches of four corners (16 16-byte loads each) ahead of their FMAs: about
-// two memory latencies per level instead of one per handful of loads.  Same
-// corners, weights and FMA order as gather_fp16: the same bits.
+// The FP16 gather: adds to acc, for levels 0..lod (NL levels when NL > 0
+// fixes the count at compile time; LV: any table with res[] and grid_h[] per
+// level), the trilinear sample of the FP16 corner-major grid: per level the 8
+// corners in the order 000, 100, 010, 110, 001, 101, 011, 111 (x fastest),
+// each an __hfma2 of its 16 channel pairs with the FP16-rounded FP32 corner
+// weight.  The corner loads are issued in two batches of four corners (16
+// 16-byte loads each) ahead of their FMAs: about two memory latencies per
+// level instead of one per handful of loads.
 template <int NL = 0, typename LV>
 __device__ __forceinline__ void gather_fp16_add(const LV& m, int lod, float px, float py, float pz,
                                                 __half2 (&acc)[kC / 2]) {
@@ -145,9 +106,9 @@ __device__ __forceinline__ void gather_fp16_add(const LV& m, int lod, float px, 
 
 // The mixed form's FP16 gather: a mixed-form ModelArgs (make_model_args)
 // holds one level, the cumulative grid of the LOD, which every mixed launch
-// gathers as lod 0, so the level loop is gone; zero on inactive lanes.  The accumulators are cleared once for both cases (a clear
-// on each side of the branch costs both on every trip).  The same bits as
-// gather_fp16(m, 0, ...).
+// gathers as lod 0, so the level count is fixed at one; zero on inactive
+// lanes.  The accumulators are cleared once for both cases (a clear on each
+// side of the busy-lane branch costs both on every trip).
 template <typename LV>
 __device__ __forceinline__ void gather_fp16_cum(const LV& m, bool active, float px, float py, float pz,
                                                 __half2 (&acc)[kC / 2]) {
@@ -319,15 +280,10 @@ __device__ __forceinline__ float decode_mixed_wave_few(const __half2 (&acc)[kC /
 }
 
 template <bool POSINV, typename LV>
-__device__ __forceinline__ float sdf_mixed_wave(const LV& m, int lod, const MixedLds* s, float b1,
-                                                uint32_t lane, bool active, float px, float py, float pz) {
+__device__ __forceinline__ float sdf_mixed_wave(const LV& m, const MixedLds* s, float b1, uint32_t lane,
+                                                bool active, float px, float py, float pz) {
     __half2 acc[kC / 2];
-    if (active) {
-        gather_fp16(m, lod, px, py, pz, acc);
-    } else {
-#pragma unroll
-        for (int c = 0; c < kC / 2; ++c) acc[c] = __float2half2_rn(0.0f);
-    }
+    gather_fp16_cum(m, active, px, py, pz, acc);
     return decode_mixed_wave<POSINV>(acc, s, b1, lane, active, px, py, pz);
 }
 
