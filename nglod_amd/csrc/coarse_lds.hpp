@@ -76,12 +76,11 @@ __device__ __forceinline__ void corner_fma_lds(float (&f)[kC], const CoarseLds32
     }
 }
 
-// gather_fp32 with levels < p.levels read from LDS: the same corners, weights
-// and FMA order, so the result is bit-identical.
+// gather_fp32's accumulation onto f (levels 0..lod) with levels < p.levels
+// read from LDS: the same corners, weights and FMA order, so the result is
+// bit-identical.
 __device__ __forceinline__ void gather_fp32_coarse(const ModelArgs& m, int lod, CoarsePlan p, const CoarseLds32* s,
                                                    float px, float py, float pz, float (&f)[kC]) {
-#pragma unroll
-    for (int c = 0; c < kC; ++c) f[c] = 0.0f;
     for (int l = 0; l <= lod; ++l) {
         const int r = m.res[l];
         const int S = r + 1;
@@ -103,7 +102,7 @@ __device__ __forceinline__ void gather_fp32_coarse(const ModelArgs& m, int lod, 
             corner_fma_lds(f, s, i0 + sz + sy + 1, wxy11 * az.w1);
         } else {
             // global levels: the corner loads in two batches of four corners
-            // ahead of their FMAs (as gather_fp32_batched; same order)
+            // ahead of their FMAs (as gather_fp32_add; same order)
             const float* base = m.grid[l] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC;
             const size_t sx = kC, sy = (size_t)S * kC, sz = (size_t)S * S * kC;
             const float* cp[8] = {base, base + sx, base + sy, base + sy + sx,

@@ -207,15 +207,6 @@ __device__ __forceinline__ void gather_fp32_add(const LV& m, int lod, float px, 
     }
 }
 
-// gather_fp32 (levels 0..lod) through gather_fp32_add.
-template <typename LV>
-__device__ __forceinline__ void gather_fp32_batched(const LV& m, int lod, float px, float py, float pz,
-                                                    float (&f)[kC]) {
-#pragma unroll
-    for (int c = 0; c < kC; ++c) f[c] = 0.0f;
-    gather_fp32_add(m, lod, px, py, pz, f);
-}
-
 // The mixed form's FP32 gather: one level, the cumulative grid of the LOD (a
 // mixed-form ModelArgs holds only that level, gathered as lod 0), zero on
 // inactive lanes, the accumulators cleared once for both cases.

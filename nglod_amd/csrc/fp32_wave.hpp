@@ -304,15 +304,15 @@ template <bool POSINV>
 __device__ __forceinline__ float sdf_fp32_wave(const ModelArgs& m, int lod, const Fp32Lds* s, float b1,
                                                uint32_t lane, bool active, float px, float py, float pz,
                                                CoarsePlan cp = CoarsePlan{0, 0}, const CoarseLds32* cs = nullptr) {
+    // cleared once for both cases (zero on inactive lanes)
     float f[kC];
+#pragma unroll
+    for (int c = 0; c < kC; ++c) f[c] = 0.0f;
     if (active) {
         if (cp.levels > 0)
             gather_fp32_coarse(m, lod, cp, cs, px, py, pz, f);
         else
-            gather_fp32_batched(m, lod, px, py, pz, f);
-    } else {
-#pragma unroll
-        for (int c = 0; c < kC; ++c) f[c] = 0.0f;
+            gather_fp32_add(m, lod, px, py, pz, f);
     }
     return decode_fp32_wave<POSINV>(f, s, b1, lane, active, px, py, pz);
 }
