@@ -206,9 +206,14 @@ __device__ __forceinline__ void add_evals(const TraceArgs& a, int slot, uint32_t
 // RESUME: a later round, over the rays pass 2 handed back (ret, rflag)
 // instead of the frame's rays.  A separate instantiation: compiled into the
 // first-round kernel, the resume path slowed its march by half even unused.
-// Up to this many busy lanes, a drained pass-1 wave gathers cooperatively
-// (gather_fp16_coop: 16 lanes per query).
+// Up to this many busy lanes, a drained wave of the FP32 pass (and of the
+// pooled march) gathers cooperatively (16 lanes per query).
 constexpr int kCoopLanes = 4;
+// Pass 1's tail: up to 8 busy lanes, 16 lanes per query up to 4 of them
+// (gather_fp16_coop) and 8 lanes per query above (gather_fp16_coop8); on
+// unless TraceArgs::solo_lanes is 0.  8 against 4 measured pass 1 0.68 ms
+// against 0.69 ms on the 1280x720 frame.
+constexpr int kPass1CoopLanes = 8;
 
 template <bool POSINV, bool RESUME>
 __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArgs a) {
@@ -227,7 +232,7 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
     uint32_t nevals = 0;
     MarchHistory hist;
     hist.clear();
-    const int coop = a.solo_lanes < kCoopLanes ? a.solo_lanes : kCoopLanes;
+    const int coop = a.solo_lanes == 0 ? 0 : kPass1CoopLanes;
     bool tail = false;
     for (;;) {
         if (RESUME) {
@@ -268,7 +273,10 @@ __global__ __launch_bounds__(kMixedBlock) void trace_mixed_pass1_kernel(TraceArg
         if (act == 0) break;
         const bool busy = s.phase != PH_IDLE;
         __half2 acc[kC / 2];
-        gather_fp16_coop(&levels, lane, act, s.qx, s.qy, s.qz, acc);
+        if (__popcll(act) <= 4)
+            gather_fp16_coop(&levels, lane, act, s.qx, s.qy, s.qz, acc);
+        else
+            gather_fp16_coop8(&levels, lane, act, s.qx, s.qy, s.qz, acc);
         const float d = decode_mixed_wave_few<POSINV>(acc, &lds, b1, lane, busy, s.qx, s.qy, s.qz);
         if (busy) {
             ++nevals;

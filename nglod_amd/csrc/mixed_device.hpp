@@ -372,4 +372,54 @@ __device__ __forceinline__ void gather_fp16_coop(const MixedLevels* lv, uint32_t
     }
 }
 
+// gather_fp16_coop for up to 8 busy lanes: 8 lanes per query, each two
+// channel pairs (lanes 8 i + j: pairs 2 j and 2 j + 1).  Same per-pair
+// __hfma2 chain, the same bits.
+__device__ __forceinline__ void gather_fp16_coop8(const MixedLevels* lv, uint32_t lane, uint64_t act, float qx,
+                                                  float qy, float qz, __half2 (&acc)[kC / 2]) {
+    const int i = (int)(lane >> 3), j = (int)(lane & 7);
+    const int k = __popcll(act);
+    const bool helper = i < k;
+    const int src = helper ? nth_set_bit(act, i) : 0;
+    const float px = __shfl(qx, src), py = __shfl(qy, src), pz = __shfl(qz, src);
+    uint2 g[8];
+    __half2 v0 = __float2half2_rn(0.0f), v1 = __float2half2_rn(0.0f);
+    if (helper) {
+        const int r = lv->res[0];
+        const int S = r + 1;
+        const AxisW ax = axis_weights(px, r);
+        const AxisW ay = axis_weights(py, r);
+        const AxisW az = axis_weights(pz, r);
+        const uint2* base = reinterpret_cast<const uint2*>(
+            lv->grid_h[0] + ((size_t)((az.i0 * S + ay.i0) * S + ax.i0)) * kC) + j;
+        const size_t sx = kC / 4, sy = (size_t)S * kC / 4, sz = (size_t)S * S * kC / 4;
+        g[0] = base[0];
+        g[1] = base[sx];
+        g[2] = base[sy];
+        g[3] = base[sy + sx];
+        g[4] = base[sz];
+        g[5] = base[sz + sx];
+        g[6] = base[sz + sy];
+        g[7] = base[sz + sy + sx];
+        const float wxy00 = ax.w0 * ay.w0, wxy10 = ax.w1 * ay.w0;
+        const float wxy01 = ax.w0 * ay.w1, wxy11 = ax.w1 * ay.w1;
+        const float w[8] = {wxy00 * az.w0, wxy10 * az.w0, wxy01 * az.w0, wxy11 * az.w0,
+                            wxy00 * az.w1, wxy10 * az.w1, wxy01 * az.w1, wxy11 * az.w1};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const __half2 wh = __float2half2_rn(w[c]);
+            v0 = __hfma2(__builtin_bit_cast(__half2, g[c].x), wh, v0);
+            v1 = __hfma2(__builtin_bit_cast(__half2, g[c].y), wh, v1);
+        }
+    }
+    const bool mine = (act >> lane) & 1ull;
+    const int q = mine ? (int)mask_rank(act) : 0;
+    const uint32_t b0 = __builtin_bit_cast(uint32_t, v0), b1 = __builtin_bit_cast(uint32_t, v1);
+#pragma unroll
+    for (int c = 0; c < kC / 2; ++c) {
+        const uint32_t w = (uint32_t)__shfl((int)((c & 1) ? b1 : b0), 8 * q + (c >> 1));
+        acc[c] = mine ? __builtin_bit_cast(__half2, w) : __float2half2_rn(0.0f);
+    }
+}
+
 }  // namespace nglod

@@ -95,7 +95,8 @@ struct TraceArgs {
     int defer_normals;        // 0: a finished hit takes its normal taps in-lane; 1: it is handed to the FP32
                               // pass (hand-off record in phase PH_N0), which takes the taps
     int solo_lanes;           // FP32 passes: a wave with no rays left to claim and at most this many busy
-                              // lanes evaluates them one at a time with the whole wave (sdf_fp32_solo)
+                              // lanes evaluates them one at a time with the whole wave (sdf_fp32_solo);
+                              // the mixed passes' cooperative tails use min(this, 4), pass 1's 8 unless 0
     int64_t nrays;  // the frame is split into kQueues ranges [nrays*q/kQueues, nrays*(q+1)/kQueues)
 };
 
