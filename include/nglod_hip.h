@@ -166,8 +166,10 @@ size_t nglod_trace_dense_workspace_size(int64_t n, int32_t form);
  * (the reference's loop trips in which the ray was active), int32 [n].
  * Environment: NGLOD_TRACE_SOLO_LANES (0..64, default 4): in the FP32
  * passes, a wave with no rays left to claim and at most this many busy lanes
- * evaluates their queries one at a time with the whole wave (same bits; 0
- * turns it off). */
+ * evaluates their queries one at a time with the whole wave (the mixed FP32
+ * pass: spreads their gather over the idle lanes, at most 4 lanes); the mixed
+ * pass 1 spreads its gather at up to 8 busy lanes unless the value is 0.  Same
+ * bits for any value; 0 turns it off. */
 int nglod_trace_dense(const nglod_model_t* model, const float* ray_o, const float* ray_d,
                       int64_t n, int32_t lod, const nglod_trace_params_t* params, int32_t form,
                       float* x, float* t, uint8_t* hit, float* normal,
