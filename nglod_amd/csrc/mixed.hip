@@ -206,8 +206,8 @@ __device__ __forceinline__ void add_evals(const TraceArgs& a, int slot, uint32_t
 // RESUME: a later round, over the rays pass 2 handed back (ret, rflag)
 // instead of the frame's rays.  A separate instantiation: compiled into the
 // first-round kernel, the resume path slowed its march by half even unused.
-// Up to this many busy lanes, a drained wave of the FP32 pass (and of the
-// pooled march) gathers cooperatively (16 lanes per query).
+// Up to this many busy lanes, a drained wave of the FP32 passes and of the
+// pooled march gathers cooperatively (16 lanes per query).
 constexpr int kCoopLanes = 4;
 // Pass 1's tail: up to 8 busy lanes, 16 lanes per query up to 4 of them
 // (gather_fp16_coop) and 8 lanes per query above (gather_fp16_coop8); on
@@ -524,13 +524,16 @@ __global__ __launch_bounds__(kMixedBlock, 3) void pool_pass1_kernel(TraceArgs a,
     const int coop = a.solo_lanes < kCoopLanes ? a.solo_lanes : kCoopLanes;
     bool tail = false;
     for (;;) {
-        for (;;) {
-            const bool need = s.phase == PH_IDLE;
-            if (__ballot(need && !feed.empty()) == 0) break;
-            const int64_t v = feed.next<kPass1Chunk>(a.queue, nullptr, lane, need);
-            if (v >= 0) {
-                rq = pool_request_of(p, v);
-                s.start(a, pool_slot_of(p, rq, v));
+        // the claim loop only when a lane needs a ray (as RayFeed::refill)
+        if (__ballot(s.phase == PH_IDLE && !feed.empty()) != 0) {
+            for (;;) {
+                const bool need = s.phase == PH_IDLE;
+                if (__ballot(need && !feed.empty()) == 0) break;
+                const int64_t v = feed.next<kPass1Chunk>(a.queue, nullptr, lane, need);
+                if (v >= 0) {
+                    rq = pool_request_of(p, v);
+                    s.start(a, pool_slot_of(p, rq, v));
+                }
             }
         }
         const uint64_t act = __ballot(s.phase != PH_IDLE);
